@@ -1,0 +1,166 @@
+/*
+ * npbcg — MI355X (gfx950) implementation of the NPB CG conj_grad path.
+ *
+ * C ABI.  Plain pointers and sizes; no C++ or torch types.  Every entry point
+ * returns 0 (NPBCG_OK) or a negative npbcg_status and never exits the process
+ * (the reference calls exit() on every failure).  A context is used by one
+ * host thread at a time and owns one HIP stream.
+ *
+ * Reference citations are GMAP/NPB-GPU file:line (CUDA/CG/cg.cu; HIP/CG/cg.cpp
+ * is the same program with hip* names).
+ */
+#ifndef NPBCG_H
+#define NPBCG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NPBCG_ABI_VERSION 1
+
+typedef enum npbcg_status {
+    NPBCG_OK = 0,
+    NPBCG_ERR_INVALID = -1,   /* bad argument: NULL pointer, n < 1, malformed CSR */
+    NPBCG_ERR_NOMEM = -2,     /* host or device allocation failed */
+    NPBCG_ERR_DEVICE = -3,    /* HIP runtime error; text via npbcg_last_error() */
+    NPBCG_ERR_NODEVICE = -4,  /* no GPU visible (cg.cu:1424-1426 exits instead) */
+    NPBCG_ERR_CAPACITY = -5,  /* nz capacity exceeded (cg.cu:1670-1674 exits instead) */
+    NPBCG_ERR_INTERNAL = -6,  /* generator insertion failure (cg.cu:1748-1751 exits) */
+    NPBCG_ERR_MISMATCH = -7,  /* arrays differ from the matrix the context holds */
+    NPBCG_ERR_NOCONV = -8     /* a solve stopped before reaching its tolerance */
+} npbcg_status;
+
+/* Inner-loop engines for the FP64 conj_grad. */
+enum {
+    NPBCG_ENGINE_AUTO = 0,
+    NPBCG_ENGINE_KERNELS = 1,     /* fused per-phase kernels, scalars stay on the GPU */
+    NPBCG_ENGINE_PERSISTENT = 2   /* one cooperative kernel runs all cgitmax iterations */
+};
+
+/* Vectors held by a context (npbcg_get_vector / npbcg_set_vector). */
+enum {
+    NPBCG_VEC_X = 0, NPBCG_VEC_Z = 1, NPBCG_VEC_P = 2, NPBCG_VEC_Q = 3,
+    NPBCG_VEC_R = 4
+};
+
+typedef struct npbcg_opts {
+    int device;       /* GPU ordinal; out of range -> 0, as GPU_DEVICE (cg.cu:1427-1432) */
+    int cgitmax;      /* inner CG iterations per conj_grad; 0 -> 25 (cg.cu:666) */
+    int engine;       /* NPBCG_ENGINE_* */
+    int spmv_variant; /* 0 = auto; 1.. = fixed SpMV kernel (see DESIGN notes in csrc) */
+    int use_graph;    /* nonzero: replay each outer iteration as a captured HIP graph */
+    int reserved[11];
+} npbcg_opts;
+
+/* Timing of the last npbcg_power / npbcg_conj_grad_device call, from HIP
+ * events recorded on the context's stream. */
+typedef struct npbcg_timing {
+    double total_ms;        /* events around the whole call */
+    double spmv_avg_ms;     /* mean duration of sampled q = A p SpMV launches */
+    int spmv_samples;       /* how many SpMV launches were bracketed (0 = none) */
+    int launches;           /* kernel launches (or graph launches) issued */
+    long long nnz;          /* rowstr[n] of the context's matrix */
+    int n;
+    int engine;             /* engine actually used */
+} npbcg_timing;
+
+void npbcg_default_opts(npbcg_opts *opts);
+/* Number of visible GPUs (the count setup_gpu reads, cg.cu:1421).  A host
+ * without a GPU yields 0 and NPBCG_OK. */
+int npbcg_device_count(int *count);
+int npbcg_abi_version(void);
+/* Text of the most recent failure in this thread ("" if none). */
+const char *npbcg_last_error(void);
+const char *npbcg_status_string(int status);
+
+/* ---- problem generator (host) -------------------------------------------
+ * Replaces makea (cg.cu:1303-1368) with sparse (:1611-1785), sprnvc
+ * (:1798-1833), vecset (:1841-1857) and the RNG set-up of main (:406-408:
+ * tran = seed, one discarded draw).  Output is the 0-based CSR after the
+ * column shift of :434-438.  rowstr/colidx/a match the reference bit for bit:
+ * duplicates are summed in the reference's order.  colidx/a need nz_cap
+ * entries; the reference's capacity is n*(nonzer+1)^2 (cg.cu:81).
+ * Returns nnz (>= 0) or a negative npbcg_status.  *ratio_out (nullable)
+ * receives pow(rcond, 1/n) (cg.cu:1695). */
+long npbcg_makea(int n, int nonzer, double rcond, double shift, double seed,
+                 long nz_cap, int *rowstr, int *colidx, double *a,
+                 double *ratio_out);
+
+/* ---- context: device residency -------------------------------------------
+ * Replaces setup_gpu (cg.cu:1390-1603): selects the device, uploads the CSR
+ * once (nnz = rowstr[n]) and allocates x, z, p, q, r (n+2 entries each, as
+ * cg.cu:105-109) on the device; x starts as all ones (cg.cu:441-443).
+ * opts may be NULL (defaults). */
+typedef struct npbcg_ctx npbcg_ctx;
+int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
+                 const double *a, const npbcg_opts *opts);
+/* Replaces release_gpu (cg.cu:1370-1388).  NULL is accepted. */
+void npbcg_destroy(npbcg_ctx *ctx);
+
+/* ---- the drop-in conj_grad -----------------------------------------------
+ * Same arguments and post-conditions as the reference's
+ *   static void conj_grad(int colidx[], int rowstr[], double x[], double z[],
+ *       double a[], double p[], double q[], double r[], double* rnorm);
+ * (cg.cu:225-233, body :653-801), with the context in front.  Host arrays:
+ * x is read (n+1 entries); z (the solution of A z = x after cgitmax CG
+ * steps), p, q, r (r = A z on exit) and *rnorm = ||x - A z|| are written,
+ * n+1 entries each; index n gets q=z=0, r=p=x[n] as cg.cu:671-676 leaves it.
+ * colidx/rowstr/a must describe the matrix the context was created with:
+ * rowstr is compared in full and colidx/a by pointer or sampled content;
+ * a mismatch returns NPBCG_ERR_MISMATCH. */
+int npbcg_conj_grad(npbcg_ctx *ctx, int colidx[], int rowstr[], double x[],
+                    double z[], double a[], double p[], double q[], double r[],
+                    double *rnorm);
+
+/* Replaces conj_grad_gpu (cg.cu:803-846): the same computation on the
+ * context's device-resident x, z, p, q, r; only *rnorm returns to the host. */
+int npbcg_conj_grad_device(npbcg_ctx *ctx, double *rnorm);
+
+/* Replaces gpu_kernel_ten (cg.cu:1182-1201): x.z and z.z of the device
+ * vectors.  Either output pointer may be NULL. */
+int npbcg_norms(npbcg_ctx *ctx, double *norm_temp1, double *norm_temp2);
+
+/* Replaces gpu_kernel_eleven (cg.cu:1247-1265): x = scale * z on the device. */
+int npbcg_scale_x(npbcg_ctx *ctx, double scale);
+
+/* The whole timed loop of cg.cu:500-520 on the device, from the context's
+ * current x: niter times { conj_grad; zeta = shift + 1/(x.z);
+ * x = z/||z|| }.  zeta_out/rnorm_out (niter entries each, nullable) receive
+ * the per-iteration values printed at cg.cu:515-516; they are copied back
+ * once at the end.  Nothing crosses to the host inside the loop. */
+int npbcg_power(npbcg_ctx *ctx, int niter, double shift, double *zeta_out,
+                double *rnorm_out);
+
+/* Device vector access (n+1 entries; index n included). */
+int npbcg_set_vector(npbcg_ctx *ctx, int which, const double *host);
+int npbcg_get_vector(npbcg_ctx *ctx, int which, double *host);
+/* x[0..n] = value, as cg.cu:441-443 and :485-487 do with 1.0. */
+int npbcg_fill_x(npbcg_ctx *ctx, double value);
+
+/* Timing of the last device call (see npbcg_timing).  Sampling SpMV launch
+ * durations adds event records to the stream; it is off unless enabled. */
+int npbcg_set_spmv_sampling(npbcg_ctx *ctx, int every_nth_call);
+int npbcg_get_timing(const npbcg_ctx *ctx, npbcg_timing *out);
+
+/* One q = A p product with the context's kernel on its device vectors,
+ * repeated reps times; *avg_ms gets the mean duration from HIP events.
+ * Used to measure SpMV bandwidth in isolation. */
+int npbcg_spmv_bench(npbcg_ctx *ctx, int reps, double *avg_ms);
+
+/* ---- reference-shaped shim ------------------------------------------------
+ * For a caller that keeps the reference's exact call
+ *   conj_grad(colidx, rowstr, x, z, a, p, q, r, &rnorm);
+ * with no context argument: the order n comes from npbcg_compat_set_n (the
+ * reference's global naa, cg.cu:124,402), and a context is created on first
+ * use and re-created if rowstr/colidx/a change.  Errors set *rnorm to NaN
+ * and leave the text in npbcg_last_error(). */
+int npbcg_compat_set_n(int naa);
+void npbcg_compat_conj_grad(int colidx[], int rowstr[], double x[], double z[],
+                            double a[], double p[], double q[], double r[],
+                            double *rnorm);
+void npbcg_compat_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NPBCG_H */

@@ -1,0 +1,628 @@
+// C ABI of npbcg: contexts, device residency, the drop-in conj_grad and the
+// device-resident inverse-power loop.  Declarations and reference citations
+// are in include/npbcg.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cg_device.h"
+#include "cg_kernels.h"
+#include "npbcg.h"
+#include "npbcg_internal.h"
+
+using npbcg_dev::Fp64Problem;
+using npbcg_dev::Scalars;
+
+namespace npbcg_detail {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+}  // namespace npbcg_detail
+
+using npbcg_detail::clear_error;
+using npbcg_detail::set_error;
+
+#define NPBCG_HIP(call)                                                          \
+    do {                                                                         \
+        hipError_t _e = (call);                                                  \
+        if (_e != hipSuccess)                                                    \
+            return set_error(NPBCG_ERR_DEVICE, std::string(#call) + ": " +       \
+                                                   hipGetErrorString(_e));       \
+    } while (0)
+
+struct npbcg_ctx {
+    int device = 0;
+    int n = 0;
+    long long nnz = 0;
+    npbcg_opts opts{};
+    int cgitmax = 25;
+    int engine = NPBCG_ENGINE_KERNELS;
+    hipStream_t stream = nullptr;
+
+    int *d_rowstr = nullptr;
+    int *d_colidx = nullptr;
+    double *d_a = nullptr;
+    double *d_vec = nullptr;   // x, z, p, q, r back to back, stride vstride
+    size_t vstride = 0;
+    double *d_part = nullptr;
+    Scalars *d_sc = nullptr;
+    double *d_hist = nullptr;  // zeta then rnorm, hist_cap each
+    int hist_cap = 0;
+    Fp64Problem pb{};
+
+    hipGraphExec_t outer_graph = nullptr;
+    hipGraph_t outer_graph_src = nullptr;
+
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::vector<hipEvent_t> probe_start, probe_stop;
+    int sample_every = 0;
+    long long device_calls = 0;
+    npbcg_timing timing{};
+
+    double *h_stage = nullptr;  // pinned host staging, 5 * (n+1) doubles + scalars
+    std::vector<int> h_rowstr;  // for the drop-in argument check
+    const int *h_colidx_ptr = nullptr;
+    const double *h_a_ptr = nullptr;
+    std::vector<int> colidx_sample;
+    std::vector<double> a_sample;
+};
+
+namespace {
+
+int pick_variant(int requested, int n, long long nnz) {
+    if (requested >= npbcg_dev::kSpmvSeg64 && requested <= npbcg_dev::kSpmvSeg64U2)
+        return requested;
+    const double avg = n > 0 ? (double)nnz / n : 0.0;
+    if (avg >= 96.0) return npbcg_dev::kSpmvSeg64;
+    if (avg >= 40.0) return npbcg_dev::kSpmvSeg32;
+    return npbcg_dev::kSpmvSeg16;
+}
+
+double *vec_ptr(npbcg_ctx *c, int which) { return c->d_vec + (size_t)which * c->vstride; }
+
+void refresh_problem(npbcg_ctx *c) {
+    Fp64Problem &pb = c->pb;
+    pb.n = c->n;
+    pb.nnz = c->nnz;
+    pb.rowstr = c->d_rowstr;
+    pb.colidx = c->d_colidx;
+    pb.a = c->d_a;
+    pb.x = vec_ptr(c, NPBCG_VEC_X);
+    pb.z = vec_ptr(c, NPBCG_VEC_Z);
+    pb.p = vec_ptr(c, NPBCG_VEC_P);
+    pb.q = vec_ptr(c, NPBCG_VEC_Q);
+    pb.r = vec_ptr(c, NPBCG_VEC_R);
+    pb.part = c->d_part;
+    pb.sc = c->d_sc;
+    pb.zeta_hist = c->d_hist;
+    pb.rnorm_hist = c->d_hist ? c->d_hist + c->hist_cap : nullptr;
+    pb.hist_cap = c->hist_cap;
+}
+
+void drop_graph(npbcg_ctx *c) {
+    if (c->outer_graph) (void)hipGraphExecDestroy(c->outer_graph);
+    if (c->outer_graph_src) (void)hipGraphDestroy(c->outer_graph_src);
+    c->outer_graph = nullptr;
+    c->outer_graph_src = nullptr;
+}
+
+int check_csr(int n, const int *rowstr, const int *colidx, long long *nnz_out) {
+    if (rowstr[0] != 0) return set_error(NPBCG_ERR_INVALID, "rowstr[0] must be 0");
+    for (int j = 0; j < n; j++)
+        if (rowstr[j + 1] < rowstr[j])
+            return set_error(NPBCG_ERR_INVALID, "rowstr must be non-decreasing");
+    const long long nnz = rowstr[n];
+    for (long long k = 0; k < nnz; k++)
+        if (colidx[k] < 0 || colidx[k] >= n)
+            return set_error(NPBCG_ERR_INVALID, "colidx entry outside [0, n)");
+    *nnz_out = nnz;
+    return NPBCG_OK;
+}
+
+// Samples used to check that the arrays handed to npbcg_conj_grad are the
+// ones the context holds (rowstr is compared in full).
+void take_samples(npbcg_ctx *c, const int *colidx, const double *a) {
+    c->colidx_sample.clear();
+    c->a_sample.clear();
+    const long long step = std::max<long long>(1, c->nnz / 61);
+    for (long long k = 0; k < c->nnz; k += step) {
+        c->colidx_sample.push_back(colidx[k]);
+        c->a_sample.push_back(a[k]);
+    }
+    if (c->nnz > 0) {
+        c->colidx_sample.push_back(colidx[c->nnz - 1]);
+        c->a_sample.push_back(a[c->nnz - 1]);
+    }
+}
+
+bool same_matrix(const npbcg_ctx *c, const int *rowstr, const int *colidx,
+                 const double *a) {
+    if (std::memcmp(rowstr, c->h_rowstr.data(), sizeof(int) * (c->n + 1)) != 0)
+        return false;
+    if (colidx == c->h_colidx_ptr && a == c->h_a_ptr) return true;
+    const long long step = std::max<long long>(1, c->nnz / 61);
+    size_t i = 0;
+    for (long long k = 0; k < c->nnz; k += step, i++)
+        if (colidx[k] != c->colidx_sample[i] ||
+            std::memcmp(&a[k], &c->a_sample[i], sizeof(double)) != 0)
+            return false;
+    return true;
+}
+
+int ensure_history(npbcg_ctx *c, int niter) {
+    if (niter <= c->hist_cap) return NPBCG_OK;
+    drop_graph(c);  // the captured graph holds the old history pointers
+    if (c->d_hist) (void)hipFree(c->d_hist);
+    c->d_hist = nullptr;
+    int cap = std::max(niter, 128);
+    NPBCG_HIP(hipMalloc(&c->d_hist, sizeof(double) * 2 * (size_t)cap));
+    c->hist_cap = cap;
+    refresh_problem(c);
+    return NPBCG_OK;
+}
+
+int ensure_graph(npbcg_ctx *c) {
+    if (c->outer_graph) return NPBCG_OK;
+    NPBCG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 1, c->stream, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return set_error(NPBCG_ERR_DEVICE,
+                         std::string("graph capture failed: ") +
+                             hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    hipGraphExec_t ex = nullptr;
+    hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return set_error(NPBCG_ERR_DEVICE,
+                         std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    c->outer_graph_src = g;
+    c->outer_graph = ex;
+    return NPBCG_OK;
+}
+
+int ensure_probes(npbcg_ctx *c) {
+    const size_t want = (size_t)c->cgitmax;
+    while (c->probe_start.size() < want) {
+        hipEvent_t a, b;
+        NPBCG_HIP(hipEventCreate(&a));
+        NPBCG_HIP(hipEventCreate(&b));
+        c->probe_start.push_back(a);
+        c->probe_stop.push_back(b);
+    }
+    return NPBCG_OK;
+}
+
+int read_scalars(npbcg_ctx *c, Scalars *out) {
+    NPBCG_HIP(hipMemcpyAsync(c->h_stage, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost,
+                             c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_stage, sizeof(Scalars));
+    return NPBCG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void npbcg_default_opts(npbcg_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->cgitmax = 25;
+    o->engine = NPBCG_ENGINE_AUTO;
+    o->spmv_variant = 0;
+    o->use_graph = 1;
+}
+
+int npbcg_abi_version(void) { return NPBCG_ABI_VERSION; }
+
+int npbcg_device_count(int *count) {
+    if (!count) return set_error(NPBCG_ERR_INVALID, "npbcg_device_count: NULL argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) ndev = 0;
+    *count = ndev;
+    return NPBCG_OK;
+}
+
+const char *npbcg_last_error(void) { return npbcg_detail::g_last_error.c_str(); }
+
+const char *npbcg_status_string(int s) {
+    switch (s) {
+    case NPBCG_OK: return "ok";
+    case NPBCG_ERR_INVALID: return "invalid argument";
+    case NPBCG_ERR_NOMEM: return "out of memory";
+    case NPBCG_ERR_DEVICE: return "HIP runtime error";
+    case NPBCG_ERR_NODEVICE: return "no GPU device";
+    case NPBCG_ERR_CAPACITY: return "matrix capacity exceeded";
+    case NPBCG_ERR_INTERNAL: return "internal generator error";
+    case NPBCG_ERR_MISMATCH: return "arrays do not match the context's matrix";
+    case NPBCG_ERR_NOCONV: return "solve did not converge";
+    default: return "unknown status";
+    }
+}
+
+int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
+                 const double *a, const npbcg_opts *opts) {
+    if (!out) return set_error(NPBCG_ERR_INVALID, "npbcg_create: out is NULL");
+    *out = nullptr;
+    if (n < 1 || !rowstr || !colidx || !a)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_create: invalid argument");
+    long long nnz = 0;
+    int rc = check_csr(n, rowstr, colidx, &nnz);
+    if (rc != NPBCG_OK) return rc;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(NPBCG_ERR_NODEVICE, "npbcg_create: no GPU device visible");
+
+    npbcg_ctx *c = new (std::nothrow) npbcg_ctx();
+    if (!c) return set_error(NPBCG_ERR_NOMEM, "npbcg_create: host allocation");
+    npbcg_default_opts(&c->opts);
+    if (opts) c->opts = *opts;
+    c->device = (c->opts.device >= 0 && c->opts.device < ndev) ? c->opts.device : 0;
+    c->cgitmax = c->opts.cgitmax > 0 ? c->opts.cgitmax : 25;
+    c->engine = NPBCG_ENGINE_KERNELS;
+    c->n = n;
+    c->nnz = nnz;
+
+    auto fail = [&](int code) {
+        npbcg_destroy(c);
+        return code;
+    };
+#define CREATE_HIP(call)                                                          \
+    do {                                                                          \
+        hipError_t _e = (call);                                                   \
+        if (_e != hipSuccess)                                                     \
+            return fail(set_error(_e == hipErrorOutOfMemory ? NPBCG_ERR_NOMEM     \
+                                                            : NPBCG_ERR_DEVICE,   \
+                                  std::string(#call) + ": " + hipGetErrorString(_e))); \
+    } while (0)
+
+    CREATE_HIP(hipSetDevice(c->device));
+    CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CREATE_HIP(hipEventCreate(&c->ev_start));
+    CREATE_HIP(hipEventCreate(&c->ev_stop));
+
+    hipDeviceProp_t prop;
+    CREATE_HIP(hipGetDeviceProperties(&prop, c->device));
+    const int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+
+    const int variant = pick_variant(c->opts.spmv_variant, n, nnz);
+    const int rpb = npbcg_dev::spmv_rows_per_block(variant);
+    const int vec_blocks = std::max(1, std::min((n + 1 + npbcg_dev::kBlock - 1) /
+                                                    npbcg_dev::kBlock, 8 * cus));
+    const int spmv_blocks = std::max(1, std::min((n + rpb - 1) / rpb, 8 * cus));
+
+    c->vstride = ((size_t)n + 2 + 63) & ~(size_t)63;
+    CREATE_HIP(hipMalloc(&c->d_rowstr, sizeof(int) * ((size_t)n + 1)));
+    CREATE_HIP(hipMalloc(&c->d_colidx, sizeof(int) * std::max<long long>(1, nnz)));
+    CREATE_HIP(hipMalloc(&c->d_a, sizeof(double) * std::max<long long>(1, nnz)));
+    CREATE_HIP(hipMalloc(&c->d_vec, sizeof(double) * 5 * c->vstride));
+    CREATE_HIP(hipMalloc(&c->d_part,
+                         sizeof(double) * 3 * (size_t)std::max(vec_blocks, spmv_blocks)));
+    CREATE_HIP(hipMalloc(&c->d_sc, npbcg_dev::scalars_bytes()));
+    CREATE_HIP(hipHostMalloc(&c->h_stage,
+                             sizeof(double) * (5 * ((size_t)n + 1)) + 4096));
+
+    CREATE_HIP(hipMemcpyAsync(c->d_rowstr, rowstr, sizeof(int) * ((size_t)n + 1),
+                              hipMemcpyHostToDevice, c->stream));
+    if (nnz > 0) {
+        CREATE_HIP(hipMemcpyAsync(c->d_colidx, colidx, sizeof(int) * nnz,
+                                  hipMemcpyHostToDevice, c->stream));
+        CREATE_HIP(hipMemcpyAsync(c->d_a, a, sizeof(double) * nnz, hipMemcpyHostToDevice,
+                                  c->stream));
+    }
+    CREATE_HIP(hipMemsetAsync(c->d_vec, 0, sizeof(double) * 5 * c->vstride, c->stream));
+    CREATE_HIP(hipMemsetAsync(c->d_sc, 0, npbcg_dev::scalars_bytes(), c->stream));
+
+    refresh_problem(c);
+    c->pb.vec_blocks = vec_blocks;
+    c->pb.spmv_blocks = spmv_blocks;
+    c->pb.spmv_variant = variant;
+    CREATE_HIP(npbcg_dev::launch_fill(c->pb.x, n + 1, 1.0, vec_blocks, c->stream));
+    CREATE_HIP(hipStreamSynchronize(c->stream));
+#undef CREATE_HIP
+
+    c->h_rowstr.assign(rowstr, rowstr + n + 1);
+    c->h_colidx_ptr = colidx;
+    c->h_a_ptr = a;
+    take_samples(c, colidx, a);
+    c->timing.n = n;
+    c->timing.nnz = nnz;
+    c->timing.engine = c->engine;
+    *out = c;
+    clear_error();
+    return NPBCG_OK;
+}
+
+void npbcg_destroy(npbcg_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (auto e : c->probe_start) (void)hipEventDestroy(e);
+    for (auto e : c->probe_stop) (void)hipEventDestroy(e);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    (void)hipFree(c->d_rowstr);
+    (void)hipFree(c->d_colidx);
+    (void)hipFree(c->d_a);
+    (void)hipFree(c->d_vec);
+    (void)hipFree(c->d_part);
+    (void)hipFree(c->d_sc);
+    (void)hipFree(c->d_hist);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int npbcg_conj_grad_device(npbcg_ctx *c, double *rnorm) {
+    if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_conj_grad_device: NULL context");
+    NPBCG_HIP(hipSetDevice(c->device));
+    NPBCG_HIP(npbcg_dev::launch_clear_pending(c->pb, c->stream));
+    NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
+    NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 0, c->stream, nullptr));
+    NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
+    Scalars s;
+    int rc = read_scalars(c, &s);
+    if (rc != NPBCG_OK) return rc;
+    float ms = 0.f;
+    NPBCG_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    c->timing.total_ms = ms;
+    c->timing.spmv_samples = 0;
+    c->timing.spmv_avg_ms = 0.0;
+    c->timing.launches = 3 * c->cgitmax + 2;
+    if (rnorm) *rnorm = s.rnorm;
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_conj_grad(npbcg_ctx *c, int colidx[], int rowstr[], double x[], double z[],
+                    double a[], double p[], double q[], double r[], double *rnorm) {
+    if (!c || !colidx || !rowstr || !x || !z || !a || !p || !q || !r || !rnorm)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_conj_grad: NULL argument");
+    if (!same_matrix(c, rowstr, colidx, a))
+        return set_error(NPBCG_ERR_MISMATCH,
+                         "npbcg_conj_grad: rowstr/colidx/a differ from the context's matrix");
+    NPBCG_HIP(hipSetDevice(c->device));
+    const size_t bytes = sizeof(double) * ((size_t)c->n + 1);
+    std::memcpy(c->h_stage, x, bytes);
+    NPBCG_HIP(hipMemcpyAsync(c->pb.x, c->h_stage, bytes, hipMemcpyHostToDevice, c->stream));
+    double rn = 0.0;
+    int rc = npbcg_conj_grad_device(c, &rn);
+    if (rc != NPBCG_OK) return rc;
+    double *outs[4] = {z, p, q, r};
+    const int which[4] = {NPBCG_VEC_Z, NPBCG_VEC_P, NPBCG_VEC_Q, NPBCG_VEC_R};
+    for (int i = 0; i < 4; i++)
+        NPBCG_HIP(hipMemcpyAsync(c->h_stage + (size_t)i * (c->n + 1), vec_ptr(c, which[i]),
+                                 bytes, hipMemcpyDeviceToHost, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 4; i++)
+        std::memcpy(outs[i], c->h_stage + (size_t)i * (c->n + 1), bytes);
+    *rnorm = rn;
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_norms(npbcg_ctx *c, double *norm_temp1, double *norm_temp2) {
+    if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_norms: NULL context");
+    NPBCG_HIP(hipSetDevice(c->device));
+    NPBCG_HIP(npbcg_dev::launch_norms(c->pb, c->stream));
+    Scalars s;
+    int rc = read_scalars(c, &s);
+    if (rc != NPBCG_OK) return rc;
+    if (norm_temp1) *norm_temp1 = s.xz;
+    if (norm_temp2) *norm_temp2 = s.zz;
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_scale_x(npbcg_ctx *c, double scale) {
+    if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_scale_x: NULL context");
+    NPBCG_HIP(hipSetDevice(c->device));
+    NPBCG_HIP(npbcg_dev::launch_scale_x(c->pb, scale, 0, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_power(npbcg_ctx *c, int niter, double shift, double *zeta_out,
+                double *rnorm_out) {
+    if (!c || niter < 0) return set_error(NPBCG_ERR_INVALID, "npbcg_power: invalid argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = ensure_history(c, std::max(niter, 1));
+    if (rc != NPBCG_OK) return rc;
+    const bool use_graph = c->opts.use_graph != 0;
+    if (use_graph) {
+        rc = ensure_graph(c);
+        if (rc != NPBCG_OK) return rc;
+    }
+    const bool sample = c->sample_every > 0 && (c->device_calls % c->sample_every) == 0;
+    c->device_calls++;
+    npbcg_dev::SpmvProbe probe{nullptr, nullptr, 0};
+    if (sample) {
+        rc = ensure_probes(c);
+        if (rc != NPBCG_OK) return rc;
+        probe = {c->probe_start.data(), c->probe_stop.data(), c->cgitmax};
+    }
+
+    NPBCG_HIP(npbcg_dev::launch_reset_scalars(c->pb, shift, c->stream));
+    NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
+    int launches = 0;
+    for (int it = 0; it < niter; it++) {
+        if (sample && it == 0) {
+            NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 1, c->stream, &probe));
+            launches += 3 * c->cgitmax + 2;
+        } else if (use_graph) {
+            NPBCG_HIP(hipGraphLaunch(c->outer_graph, c->stream));
+            launches += 1;
+        } else {
+            NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 1, c->stream, nullptr));
+            launches += 3 * c->cgitmax + 2;
+        }
+    }
+    NPBCG_HIP(npbcg_dev::launch_scale_x(c->pb, 0.0, 1, c->stream));
+    NPBCG_HIP(npbcg_dev::launch_clear_pending(c->pb, c->stream));
+    NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    if (niter > 0 && zeta_out)
+        NPBCG_HIP(hipMemcpy(zeta_out, c->d_hist, sizeof(double) * niter,
+                            hipMemcpyDeviceToHost));
+    if (niter > 0 && rnorm_out)
+        NPBCG_HIP(hipMemcpy(rnorm_out, c->d_hist + c->hist_cap, sizeof(double) * niter,
+                            hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    NPBCG_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    c->timing.total_ms = ms;
+    c->timing.launches = launches + 2;
+    c->timing.spmv_samples = 0;
+    c->timing.spmv_avg_ms = 0.0;
+    if (sample && niter > 0) {
+        double sum = 0.0;
+        for (int i = 0; i < probe.count; i++) {
+            float t = 0.f;
+            NPBCG_HIP(hipEventElapsedTime(&t, c->probe_start[i], c->probe_stop[i]));
+            sum += t;
+        }
+        c->timing.spmv_samples = probe.count;
+        c->timing.spmv_avg_ms = sum / probe.count;
+    }
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_set_vector(npbcg_ctx *c, int which, const double *host) {
+    if (!c || !host || which < NPBCG_VEC_X || which > NPBCG_VEC_R)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_set_vector: invalid argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    const size_t bytes = sizeof(double) * ((size_t)c->n + 1);
+    std::memcpy(c->h_stage, host, bytes);
+    NPBCG_HIP(hipMemcpyAsync(vec_ptr(c, which), c->h_stage, bytes, hipMemcpyHostToDevice,
+                             c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_get_vector(npbcg_ctx *c, int which, double *host) {
+    if (!c || !host || which < NPBCG_VEC_X || which > NPBCG_VEC_R)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_get_vector: invalid argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    const size_t bytes = sizeof(double) * ((size_t)c->n + 1);
+    NPBCG_HIP(hipMemcpyAsync(c->h_stage, vec_ptr(c, which), bytes, hipMemcpyDeviceToHost,
+                             c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(host, c->h_stage, bytes);
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_fill_x(npbcg_ctx *c, double value) {
+    if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_fill_x: NULL context");
+    NPBCG_HIP(hipSetDevice(c->device));
+    NPBCG_HIP(npbcg_dev::launch_fill(c->pb.x, c->n + 1, value, c->pb.vec_blocks, c->stream));
+    NPBCG_HIP(npbcg_dev::launch_clear_pending(c->pb, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_set_spmv_sampling(npbcg_ctx *c, int every_nth_call) {
+    if (!c || every_nth_call < 0)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_set_spmv_sampling: invalid argument");
+    c->sample_every = every_nth_call;
+    c->device_calls = 0;
+    return NPBCG_OK;
+}
+
+int npbcg_get_timing(const npbcg_ctx *c, npbcg_timing *out) {
+    if (!c || !out) return set_error(NPBCG_ERR_INVALID, "npbcg_get_timing: NULL argument");
+    *out = c->timing;
+    return NPBCG_OK;
+}
+
+int npbcg_spmv_bench(npbcg_ctx *c, int reps, double *avg_ms) {
+    if (!c || reps < 1 || !avg_ms)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_spmv_bench: invalid argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    NPBCG_HIP(npbcg_dev::launch_spmv_pq(c->pb, c->stream));  // warm
+    NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
+    for (int i = 0; i < reps; i++) NPBCG_HIP(npbcg_dev::launch_spmv_pq(c->pb, c->stream));
+    NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    NPBCG_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    *avg_ms = ms / reps;
+    clear_error();
+    return NPBCG_OK;
+}
+
+// ---- reference-shaped shim -------------------------------------------------
+
+namespace {
+std::mutex g_compat_mu;
+int g_compat_n = 0;
+npbcg_ctx *g_compat_ctx = nullptr;
+}  // namespace
+
+int npbcg_compat_set_n(int naa) {
+    if (naa < 1) return set_error(NPBCG_ERR_INVALID, "npbcg_compat_set_n: naa < 1");
+    std::lock_guard<std::mutex> lk(g_compat_mu);
+    if (naa != g_compat_n && g_compat_ctx) {
+        npbcg_destroy(g_compat_ctx);
+        g_compat_ctx = nullptr;
+    }
+    g_compat_n = naa;
+    return NPBCG_OK;
+}
+
+void npbcg_compat_conj_grad(int colidx[], int rowstr[], double x[], double z[], double a[],
+                            double p[], double q[], double r[], double *rnorm) {
+    std::lock_guard<std::mutex> lk(g_compat_mu);
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    if (g_compat_n < 1 || !rowstr) {
+        set_error(NPBCG_ERR_INVALID, "npbcg_compat_conj_grad: call npbcg_compat_set_n first");
+        if (rnorm) *rnorm = nan;
+        return;
+    }
+    if (g_compat_ctx && !same_matrix(g_compat_ctx, rowstr, colidx, a)) {
+        npbcg_destroy(g_compat_ctx);
+        g_compat_ctx = nullptr;
+    }
+    if (!g_compat_ctx) {
+        if (npbcg_create(&g_compat_ctx, g_compat_n, rowstr, colidx, a, nullptr) != NPBCG_OK) {
+            g_compat_ctx = nullptr;
+            if (rnorm) *rnorm = nan;
+            return;
+        }
+    }
+    if (npbcg_conj_grad(g_compat_ctx, colidx, rowstr, x, z, a, p, q, r, rnorm) != NPBCG_OK)
+        if (rnorm) *rnorm = nan;
+}
+
+void npbcg_compat_release(void) {
+    std::lock_guard<std::mutex> lk(g_compat_mu);
+    npbcg_destroy(g_compat_ctx);
+    g_compat_ctx = nullptr;
+}
+
+}  // extern "C"

@@ -1,0 +1,244 @@
+"""Python bindings (ctypes) for libnpbcg, the gfx950 NPB CG conj_grad library.
+
+The compute path is the HIP library; this module only marshals arrays across
+the C ABI declared in ``include/npbcg.h``.  Importing it raises if the built
+library is absent — there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnpbcg.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "npbcg.h")
+
+OK = 0
+ERR_INVALID = -1
+ERR_NOMEM = -2
+ERR_DEVICE = -3
+ERR_NODEVICE = -4
+ERR_CAPACITY = -5
+ERR_INTERNAL = -6
+ERR_MISMATCH = -7
+ERR_NOCONV = -8
+
+ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
+VEC_X, VEC_Z, VEC_P, VEC_Q, VEC_R = 0, 1, 2, 3, 4
+SPMV_AUTO, SPMV_SEG64, SPMV_SEG32, SPMV_SEG16, SPMV_SEG64_U2 = 0, 1, 2, 3, 4
+
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+class NpbcgError(RuntimeError):
+    def __init__(self, code: int, where: str, text: str):
+        super().__init__(f"{where}: status {code}: {text}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("cgitmax", ctypes.c_int),
+                ("engine", ctypes.c_int), ("spmv_variant", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("reserved", ctypes.c_int * 11)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("spmv_avg_ms", ctypes.c_double),
+                ("spmv_samples", ctypes.c_int), ("launches", ctypes.c_int),
+                ("nnz", ctypes.c_longlong), ("n", ctypes.c_int), ("engine", ctypes.c_int)]
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libnpbcg.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "npbcg_default_opts": (None, [ctypes.POINTER(Opts)]),
+        "npbcg_abi_version": (ctypes.c_int, []),
+        "npbcg_last_error": (ctypes.c_char_p, []),
+        "npbcg_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "npbcg_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+        "npbcg_makea": (ctypes.c_long, [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_long,
+                                        _I32P, _I32P, _F64P, _DP]),
+        "npbcg_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, _I32P, _I32P,
+                                        _F64P, ctypes.POINTER(Opts)]),
+        "npbcg_destroy": (None, [vp]),
+        "npbcg_conj_grad": (ctypes.c_int, [vp, _I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
+                                           _F64P, _F64P, _DP]),
+        "npbcg_conj_grad_device": (ctypes.c_int, [vp, _DP]),
+        "npbcg_norms": (ctypes.c_int, [vp, _DP, _DP]),
+        "npbcg_scale_x": (ctypes.c_int, [vp, ctypes.c_double]),
+        "npbcg_power": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, _F64P, _F64P]),
+        "npbcg_set_vector": (ctypes.c_int, [vp, ctypes.c_int, _F64P]),
+        "npbcg_get_vector": (ctypes.c_int, [vp, ctypes.c_int, _F64P]),
+        "npbcg_fill_x": (ctypes.c_int, [vp, ctypes.c_double]),
+        "npbcg_set_spmv_sampling": (ctypes.c_int, [vp, ctypes.c_int]),
+        "npbcg_get_timing": (ctypes.c_int, [vp, ctypes.POINTER(Timing)]),
+        "npbcg_spmv_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
+        "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
+        "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
+                                          _F64P, _F64P, _DP]),
+        "npbcg_compat_release": (None, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load()
+
+
+def header_functions() -> list[str]:
+    """Names of the functions include/npbcg.h declares."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(npbcg_[a-z0-9_]+)\s*\(", text)))
+
+
+def last_error() -> str:
+    return (LIB.npbcg_last_error() or b"").decode()
+
+
+def _check(code: int, where: str) -> int:
+    if code < 0:
+        raise NpbcgError(code, where, last_error() or LIB.npbcg_status_string(code).decode())
+    return code
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _check(LIB.npbcg_device_count(ctypes.byref(c)), "npbcg_device_count")
+    return c.value
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    LIB.npbcg_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Matrix:
+    """An NPB CG matrix in 0-based CSR (int32 indices, float64 values)."""
+
+    def __init__(self, n, rowstr, colidx, a, ratio=None, nonzer=None, shift=None,
+                 rcond=None, seed=None):
+        self.n, self.rowstr, self.colidx, self.a = n, rowstr, colidx, a
+        self.ratio, self.nonzer, self.shift, self.rcond, self.seed = (
+            ratio, nonzer, shift, rcond, seed)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowstr[self.n])
+
+
+def makea(n: int, nonzer: int, shift: float, seed: float = 314159265.0,
+          rcond: float = 0.1, nz_cap: Optional[int] = None) -> Matrix:
+    """The NPB CG generator (bit-exact with the reference's makea)."""
+    cap = n * (nonzer + 1) * (nonzer + 1) if nz_cap is None else nz_cap
+    rowstr = np.zeros(n + 1, dtype=np.int32)
+    colidx = np.zeros(max(cap, 1), dtype=np.int32)
+    a = np.zeros(max(cap, 1), dtype=np.float64)
+    ratio = ctypes.c_double(0.0)
+    nnz = _check(LIB.npbcg_makea(n, nonzer, rcond, shift, float(seed), cap, rowstr,
+                                 colidx, a, ctypes.byref(ratio)), "npbcg_makea")
+    return Matrix(n, rowstr, colidx[:nnz].copy(), a[:nnz].copy(), ratio.value, nonzer,
+                  shift, rcond, seed)
+
+
+class Context:
+    """A device-resident problem (npbcg_create .. npbcg_destroy)."""
+
+    def __init__(self, m: Matrix, **opts):
+        self.m = m
+        self.n = m.n
+        self._h = ctypes.c_void_p()
+        o = default_opts(**opts)
+        _check(LIB.npbcg_create(ctypes.byref(self._h), m.n, m.rowstr, m.colidx, m.a,
+                                ctypes.byref(o)), "npbcg_create")
+
+    def close(self):
+        if self._h:
+            LIB.npbcg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def conj_grad(self, x: np.ndarray, colidx=None, rowstr=None, a=None):
+        """Drop-in conj_grad on host arrays; returns (z, p, q, r, rnorm)."""
+        n = self.n
+        x = np.ascontiguousarray(x[: n + 1], dtype=np.float64)
+        z, p, q, r = (np.zeros(n + 1) for _ in range(4))
+        rn = ctypes.c_double(0.0)
+        _check(LIB.npbcg_conj_grad(self._h,
+                                   self.m.colidx if colidx is None else colidx,
+                                   self.m.rowstr if rowstr is None else rowstr,
+                                   x, z, self.m.a if a is None else a, p, q, r,
+                                   ctypes.byref(rn)), "npbcg_conj_grad")
+        return z, p, q, r, rn.value
+
+    def conj_grad_device(self) -> float:
+        rn = ctypes.c_double(0.0)
+        _check(LIB.npbcg_conj_grad_device(self._h, ctypes.byref(rn)), "npbcg_conj_grad_device")
+        return rn.value
+
+    def norms(self):
+        t1, t2 = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(LIB.npbcg_norms(self._h, ctypes.byref(t1), ctypes.byref(t2)), "npbcg_norms")
+        return t1.value, t2.value
+
+    def scale_x(self, s: float):
+        _check(LIB.npbcg_scale_x(self._h, s), "npbcg_scale_x")
+
+    def power(self, niter: int, shift: float):
+        zeta = np.zeros(max(niter, 1))
+        rnorm = np.zeros(max(niter, 1))
+        _check(LIB.npbcg_power(self._h, niter, shift, zeta, rnorm), "npbcg_power")
+        return zeta[:niter], rnorm[:niter]
+
+    def set_vector(self, which: int, v: np.ndarray):
+        v = np.ascontiguousarray(v[: self.n + 1], dtype=np.float64)
+        _check(LIB.npbcg_set_vector(self._h, which, v), "npbcg_set_vector")
+
+    def get_vector(self, which: int) -> np.ndarray:
+        out = np.zeros(self.n + 1)
+        _check(LIB.npbcg_get_vector(self._h, which, out), "npbcg_get_vector")
+        return out
+
+    def fill_x(self, value: float = 1.0):
+        _check(LIB.npbcg_fill_x(self._h, value), "npbcg_fill_x")
+
+    def set_spmv_sampling(self, every: int):
+        _check(LIB.npbcg_set_spmv_sampling(self._h, every), "npbcg_set_spmv_sampling")
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check(LIB.npbcg_get_timing(self._h, ctypes.byref(t)), "npbcg_get_timing")
+        return t
+
+    def spmv_bench(self, reps: int = 20) -> float:
+        ms = ctypes.c_double(0.0)
+        _check(LIB.npbcg_spmv_bench(self._h, reps, ctypes.byref(ms)), "npbcg_spmv_bench")
+        return ms.value

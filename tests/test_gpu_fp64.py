@@ -1,0 +1,194 @@
+"""GPU parity of the FP64 conj_grad path (the HIP library through its C ABI)
+against the CPU oracle and against the reference's own trajectories.
+
+Tolerances (FP64; the GPU sums in a different, fixed order):
+  * zeta: relative 1e-12 per outer iteration (the target the path is held to);
+  * vectors z, p, q, r: max abs difference <= 1e-12 * max(1, max|x|, max|v|);
+  * rnorm: abs difference <= 1e-12 * ||x||.
+NPB verification (relative 1e-10 to the published zeta) is checked as well.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+ZETA_RTOL = 1e-12
+VEC_TOL = 1e-12
+
+
+def _ocsr(m):
+    return oracle.CSR(m.n, m.rowstr, m.colidx, m.a, m.ratio, m.nonzer, m.shift, m.rcond,
+                      m.seed)
+
+
+def _gen(lib, c):
+    return lib.makea(c["n"], c["nonzer"], c["shift"], c["seed"], c["rcond"])
+
+
+def _vec_close(v, ref, xmax, label):
+    scale = max(1.0, xmax, float(np.max(np.abs(ref))))
+    err = float(np.max(np.abs(v - ref)))
+    print(f"{label}: max abs err {err:.3e} (scale {scale:.3e})")
+    assert err <= VEC_TOL * scale, label
+
+
+def _zeta_close(z, ref, label):
+    z, ref = np.asarray(z), np.asarray(ref)
+    rel = np.abs(z - ref) / np.abs(ref)
+    print(f"{label}: max rel zeta err {rel.max():.3e}")
+    assert np.all(rel <= ZETA_RTOL), label
+
+
+@pytest.mark.parametrize("name", ["S_npb", "cfg1_n1400_seed12345", "W_npb"])
+def test_conj_grad_dropin_matches_oracle(gpu, golden, name):
+    c = golden[name]
+    m = _gen(gpu, c)
+    n = m.n
+    rng = np.random.default_rng(7)
+    x = np.ones(n + 2)
+    x[:n] += 0.25 * rng.standard_normal(n)
+    x[n] = 3.0
+    with gpu.Context(m) as ctx:
+        z, p, q, r, rn = ctx.conj_grad(x)
+    oz, op, oq, orr, orn = oracle.conj_grad(_ocsr(m), x)
+    xmax = float(np.max(np.abs(x[:n])))
+    for v, ref, lab in [(z, oz, "z"), (p, op, "p"), (q, oq, "q"), (r, orr, "r")]:
+        _vec_close(v[:n], ref[:n], xmax, f"{name} {lab}")
+    assert abs(rn - orn) <= 1e-12 * np.linalg.norm(x[:n])
+    # index n: q = z = 0, r = p = x[n] (cg.cu:671-676)
+    assert z[n] == 0.0 and q[n] == 0.0 and r[n] == 3.0 and p[n] == 3.0
+
+
+@pytest.mark.parametrize("name", ["S_npb", "cfg1_n1400_seed12345", "W_npb", "A_npb"])
+def test_power_matches_oracle_and_reference(gpu, golden, name):
+    c = golden[name]
+    m = _gen(gpu, c)
+    niter = c["niter"]
+    with gpu.Context(m) as ctx:
+        zeta, rnorm = ctx.power(niter, c["shift"])
+        xg = ctx.get_vector(gpu.VEC_X)
+    ozeta, ornorm, ox = oracle.power(_ocsr(m), niter)
+    _zeta_close(zeta, ozeta, f"{name} vs oracle")
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"]], f"{name} vs reference")
+    assert np.all(np.abs(rnorm - ornorm) <= 1e-12 * np.sqrt(m.n))
+    _vec_close(xg[: m.n], ox[: m.n], 1.0, f"{name} final x")
+    assert xg[m.n] == 1.0
+    if name.endswith("_npb"):
+        zref = oracle.NPB_CLASSES[c["npb_class"]][4]
+        assert abs(zeta[-1] - zref) / zref <= oracle.NPB_EPSILON
+
+
+@pytest.mark.parametrize("name", ["B_npb", "cfg2_n75000_seed271828", "C_npb",
+                                  "cfg3_n150000_seed314"])
+def test_power_large_matches_reference(gpu, golden, name):
+    c = golden[name]
+    m = _gen(gpu, c)
+    with gpu.Context(m) as ctx:
+        zeta, rnorm = ctx.power(c["niter"], c["shift"])
+    ref = [float.fromhex(h) for h in c["zeta_hex"]]
+    _zeta_close(zeta, ref, f"{name} vs reference")
+    rref = np.array([float.fromhex(h) for h in c["rnorm_hex"]])
+    print(f"{name}: max abs rnorm err {np.max(np.abs(rnorm - rref)):.3e}")
+    assert np.all(np.abs(rnorm - rref) <= 1e-12 * np.sqrt(m.n))
+    if name.endswith("_npb"):
+        zref = oracle.NPB_CLASSES[c["npb_class"]][4]
+        assert abs(zeta[-1] - zref) / zref <= oracle.NPB_EPSILON
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("name", ["cfg1_n1400_seed12345", "A_npb"])
+def test_spmv_variants(gpu, golden, name, variant):
+    c = golden[name]
+    m = _gen(gpu, c)
+    rng = np.random.default_rng(variant)
+    v = np.zeros(m.n + 1)
+    v[: m.n] = rng.standard_normal(m.n)
+    with gpu.Context(m, spmv_variant=variant) as ctx:
+        ctx.set_vector(gpu.VEC_P, v)
+        ctx.spmv_bench(1)
+        q = ctx.get_vector(gpu.VEC_Q)
+        zeta, _ = ctx.power(3, c["shift"])
+    _vec_close(q[: m.n], oracle.spmv(_ocsr(m), v[: m.n]), float(np.max(np.abs(v))),
+               f"{name} spmv variant {variant}")
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"][:3]],
+                f"{name} variant {variant}")
+
+
+def test_power_is_deterministic_and_graph_matches_direct(gpu, golden):
+    c = golden["A_npb"]
+    m = _gen(gpu, c)
+    with gpu.Context(m) as g, gpu.Context(m, use_graph=0) as d:
+        z1, r1 = g.power(6, c["shift"])
+        g.fill_x(1.0)
+        z2, r2 = g.power(6, c["shift"])
+        z3, r3 = d.power(6, c["shift"])
+    assert z1.tobytes() == z2.tobytes() and r1.tobytes() == r2.tobytes()
+    assert z1.tobytes() == z3.tobytes() and r1.tobytes() == r3.tobytes()
+
+
+def test_device_steps_reproduce_power_loop(gpu, golden):
+    # conj_grad_gpu + gpu_kernel_ten + gpu_kernel_eleven, as cg.cu:500-520 calls them
+    c = golden["S_npb"]
+    m = _gen(gpu, c)
+    with gpu.Context(m) as ctx:
+        zs = []
+        for _ in range(4):
+            ctx.conj_grad_device()
+            t1, t2 = ctx.norms()
+            zs.append(c["shift"] + 1.0 / t1)
+            ctx.scale_x(1.0 / np.sqrt(t2))
+    ref = [float.fromhex(h) for h in c["zeta_hex"][:4]]
+    _zeta_close(zs, ref, "device steps")
+
+
+def test_compat_shim_reference_signature(gpu, golden):
+    import ctypes
+
+    c = golden["cfg1_n1400_seed12345"]
+    m = _gen(gpu, c)
+    n = m.n
+    x = np.ones(n + 1)
+    z, p, q, r = (np.zeros(n + 1) for _ in range(4))
+    rn = ctypes.c_double(0.0)
+    assert gpu.LIB.npbcg_compat_set_n(n) == 0
+    gpu.LIB.npbcg_compat_conj_grad(m.colidx, m.rowstr, x, z, m.a, p, q, r, ctypes.byref(rn))
+    gpu.LIB.npbcg_compat_release()
+    oz, _, _, _, orn = oracle.conj_grad(_ocsr(m), np.ones(n + 2))
+    _vec_close(z[:n], oz[:n], 1.0, "compat z")
+    assert abs(rn.value - orn) <= 1e-12 * np.sqrt(n)
+
+
+def test_mismatched_arrays_are_rejected(gpu, golden):
+    c = golden["cfg1_n1400_seed12345"]
+    m = _gen(gpu, c)
+    other = gpu.makea(m.n, 7, 10.0, 99.0)
+    with gpu.Context(m) as ctx:
+        x = np.ones(m.n + 1)
+        with pytest.raises(gpu.NpbcgError) as e:
+            ctx.conj_grad(x, colidx=other.colidx, rowstr=other.rowstr, a=other.a)
+        assert e.value.code == gpu.ERR_MISMATCH
+        a2 = m.a.copy()
+        a2[::7] *= 2.0
+        with pytest.raises(gpu.NpbcgError):
+            ctx.conj_grad(x, a=a2)
+        ctx.conj_grad(x, colidx=m.colidx.copy(), rowstr=m.rowstr.copy(), a=m.a.copy())
+
+
+def test_vector_roundtrip_and_sampling(gpu, golden):
+    c = golden["S_npb"]
+    m = _gen(gpu, c)
+    with gpu.Context(m) as ctx:
+        v = np.arange(m.n + 1, dtype=np.float64)
+        for w in range(5):
+            ctx.set_vector(w, v + w)
+            assert np.array_equal(ctx.get_vector(w), v + w)
+        ctx.fill_x(1.0)
+        ctx.set_spmv_sampling(1)
+        zeta, _ = ctx.power(2, c["shift"])
+        t = ctx.timing()
+        assert t.spmv_samples == 25 and t.spmv_avg_ms > 0 and t.total_ms > 0
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"][:2]], "sampled run")
