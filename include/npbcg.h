@@ -47,9 +47,12 @@ typedef struct npbcg_opts {
     int device;       /* GPU ordinal; out of range -> 0, as GPU_DEVICE (cg.cu:1427-1432) */
     int cgitmax;      /* inner CG iterations per conj_grad; 0 -> 25 (cg.cu:666) */
     int engine;       /* NPBCG_ENGINE_* */
-    int spmv_variant; /* 0 = auto; 1.. = fixed SpMV kernel (see DESIGN notes in csrc) */
+    int spmv_variant; /* 0 = auto; 1-4 CSR row kernels (64/32/16 lanes per row,
+                         64 with 2 loads in flight); 5 = LDS-staged column blocks */
     int use_graph;    /* nonzero: replay each outer iteration as a captured HIP graph */
-    int reserved[11];
+    int tile_rows;    /* LDS-staged SpMV: rows per tile, 0 = auto */
+    int tile_width;   /* LDS-staged SpMV: vector entries per LDS block, 0 = auto */
+    int reserved[9];
 } npbcg_opts;
 
 /* Timing of the last npbcg_power / npbcg_conj_grad_device call, from HIP

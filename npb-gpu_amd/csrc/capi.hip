@@ -14,6 +14,7 @@
 #include "cg_kernels.h"
 #include "npbcg.h"
 #include "npbcg_internal.h"
+#include "tiled_layout.h"
 
 using npbcg_dev::Fp64Problem;
 using npbcg_dev::Scalars;
@@ -56,6 +57,10 @@ struct npbcg_ctx {
     int *d_rowstr = nullptr;
     int *d_colidx = nullptr;
     double *d_a = nullptr;
+    int *d_tseg = nullptr;         // column-blocked copy (tiled SpMV)
+    double *d_tval = nullptr;
+    uint16_t *d_tcol = nullptr;
+    bool cols_sorted = true;
     double *d_vec = nullptr;   // x, z, p, q, r back to back, stride vstride
     size_t vstride = 0;
     double *d_part = nullptr;
@@ -83,10 +88,13 @@ struct npbcg_ctx {
 
 namespace {
 
-int pick_variant(int requested, int n, long long nnz) {
-    if (requested >= npbcg_dev::kSpmvSeg64 && requested <= npbcg_dev::kSpmvSeg64U2)
+int pick_variant(int requested, int n, long long nnz, bool sorted) {
+    if (requested >= npbcg_dev::kSpmvSeg64 && requested <= npbcg_dev::kSpmvTiled) {
+        if (requested == npbcg_dev::kSpmvTiled && !sorted) return npbcg_dev::kSpmvSeg64;
         return requested;
+    }
     const double avg = n > 0 ? (double)nnz / n : 0.0;
+    if (sorted && n >= 16384 && avg >= 16.0) return npbcg_dev::kSpmvTiled;
     if (avg >= 96.0) return npbcg_dev::kSpmvSeg64;
     if (avg >= 40.0) return npbcg_dev::kSpmvSeg32;
     return npbcg_dev::kSpmvSeg16;
@@ -120,16 +128,22 @@ void drop_graph(npbcg_ctx *c) {
     c->outer_graph_src = nullptr;
 }
 
-int check_csr(int n, const int *rowstr, const int *colidx, long long *nnz_out) {
+int check_csr(int n, const int *rowstr, const int *colidx, long long *nnz_out,
+              bool *sorted_out) {
     if (rowstr[0] != 0) return set_error(NPBCG_ERR_INVALID, "rowstr[0] must be 0");
     for (int j = 0; j < n; j++)
         if (rowstr[j + 1] < rowstr[j])
             return set_error(NPBCG_ERR_INVALID, "rowstr must be non-decreasing");
     const long long nnz = rowstr[n];
-    for (long long k = 0; k < nnz; k++)
-        if (colidx[k] < 0 || colidx[k] >= n)
-            return set_error(NPBCG_ERR_INVALID, "colidx entry outside [0, n)");
+    bool sorted = true;
+    for (int j = 0; j < n; j++)
+        for (long long k = rowstr[j]; k < rowstr[j + 1]; k++) {
+            if (colidx[k] < 0 || colidx[k] >= n)
+                return set_error(NPBCG_ERR_INVALID, "colidx entry outside [0, n)");
+            if (k > rowstr[j] && colidx[k] <= colidx[k - 1]) sorted = false;
+        }
     *nnz_out = nnz;
+    *sorted_out = sorted;
     return NPBCG_OK;
 }
 
@@ -267,7 +281,8 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     if (n < 1 || !rowstr || !colidx || !a)
         return set_error(NPBCG_ERR_INVALID, "npbcg_create: invalid argument");
     long long nnz = 0;
-    int rc = check_csr(n, rowstr, colidx, &nnz);
+    bool sorted = true;
+    int rc = check_csr(n, rowstr, colidx, &nnz, &sorted);
     if (rc != NPBCG_OK) return rc;
 
     int ndev = 0;
@@ -283,6 +298,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     c->engine = NPBCG_ENGINE_KERNELS;
     c->n = n;
     c->nnz = nnz;
+    c->cols_sorted = sorted;
 
     auto fail = [&](int code) {
         npbcg_destroy(c);
@@ -306,11 +322,46 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     CREATE_HIP(hipGetDeviceProperties(&prop, c->device));
     const int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
 
-    const int variant = pick_variant(c->opts.spmv_variant, n, nnz);
+    int variant = pick_variant(c->opts.spmv_variant, n, nnz, sorted);
     const int rpb = npbcg_dev::spmv_rows_per_block(variant);
     const int vec_blocks = std::max(1, std::min((n + 1 + npbcg_dev::kBlock - 1) /
                                                     npbcg_dev::kBlock, 8 * cus));
     const int spmv_blocks = std::max(1, std::min((n + rpb - 1) / rpb, 8 * cus));
+
+    // Column-blocked copy for the LDS-staged SpMV: one 1024-thread workgroup
+    // per tile, the tiles spread evenly over the CUs, W vector entries per
+    // LDS block (W * 8 bytes + T running sums must fit the CU's LDS).
+    npbcg_detail::TiledMatrix<double> tm;
+    npbcg_dev::TiledDev td;
+    if (variant == npbcg_dev::kSpmvTiled) {
+        int lds_cap = 0;
+        if (hipDeviceGetAttribute(&lds_cap, hipDeviceAttributeSharedMemPerBlockOptin,
+                                  c->device) != hipSuccess || lds_cap <= 0)
+            lds_cap = 64 * 1024;
+        const int lds_budget = lds_cap - 2048;  // static LDS of the reduction epilogue
+        int W = c->opts.tile_width > 0 ? c->opts.tile_width
+                                       : std::min(16384, (lds_budget / 8) * 3 / 4);
+        W = std::max(2, W & ~1);
+        if (W > n) W = (n + 1) & ~1;
+        const int per_cu = std::max(1, (int)((n + (long long)cus * 2048 - 1) /
+                                             ((long long)cus * 2048)));
+        const int T = c->opts.tile_rows > 0
+                          ? c->opts.tile_rows
+                          : std::max(1, (n + cus * per_cu - 1) / (cus * per_cu));
+        if ((W + T) * 8 > lds_budget ||
+            !npbcg_detail::build_tiled(n, rowstr, colidx, a, T, W, &tm)) {
+            variant = pick_variant(npbcg_dev::kSpmvSeg64, n, nnz, sorted);
+        } else {
+            td.T = tm.T;
+            td.W = tm.W;
+            td.nblk = tm.nblk;
+            td.ntile = tm.ntile;
+            td.grid = tm.ntile;
+            td.lds_bytes = (tm.W + tm.T) * 8;
+            const double seg_len = (double)nnz / ((double)n * tm.nblk);
+            td.seg_lanes = seg_len >= 48.0 ? 64 : (seg_len >= 20.0 ? 32 : 16);
+        }
+    }
 
     c->vstride = ((size_t)n + 2 + 63) & ~(size_t)63;
     CREATE_HIP(hipMalloc(&c->d_rowstr, sizeof(int) * ((size_t)n + 1)));
@@ -318,7 +369,25 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     CREATE_HIP(hipMalloc(&c->d_a, sizeof(double) * std::max<long long>(1, nnz)));
     CREATE_HIP(hipMalloc(&c->d_vec, sizeof(double) * 5 * c->vstride));
     CREATE_HIP(hipMalloc(&c->d_part,
-                         sizeof(double) * 3 * (size_t)std::max(vec_blocks, spmv_blocks)));
+                         sizeof(double) * 3 *
+                             (size_t)std::max({vec_blocks, spmv_blocks, td.grid})));
+    if (variant == npbcg_dev::kSpmvTiled) {
+        CREATE_HIP(npbcg_dev::prepare_tiled_kernels(td.lds_bytes));
+        CREATE_HIP(hipMalloc(&c->d_tseg, sizeof(int) * tm.seg.size()));
+        CREATE_HIP(hipMalloc(&c->d_tval, sizeof(double) * std::max<size_t>(1, tm.val.size())));
+        CREATE_HIP(hipMalloc(&c->d_tcol, sizeof(uint16_t) * std::max<size_t>(1, tm.col.size())));
+        CREATE_HIP(hipMemcpy(c->d_tseg, tm.seg.data(), sizeof(int) * tm.seg.size(),
+                             hipMemcpyHostToDevice));
+        if (!tm.val.empty()) {
+            CREATE_HIP(hipMemcpy(c->d_tval, tm.val.data(), sizeof(double) * tm.val.size(),
+                                 hipMemcpyHostToDevice));
+            CREATE_HIP(hipMemcpy(c->d_tcol, tm.col.data(), sizeof(uint16_t) * tm.col.size(),
+                                 hipMemcpyHostToDevice));
+        }
+        td.seg = c->d_tseg;
+        td.val = c->d_tval;
+        td.col = c->d_tcol;
+    }
     CREATE_HIP(hipMalloc(&c->d_sc, npbcg_dev::scalars_bytes()));
     CREATE_HIP(hipHostMalloc(&c->h_stage,
                              sizeof(double) * (5 * ((size_t)n + 1)) + 4096));
@@ -338,6 +407,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     c->pb.vec_blocks = vec_blocks;
     c->pb.spmv_blocks = spmv_blocks;
     c->pb.spmv_variant = variant;
+    c->pb.tiled = td;
     CREATE_HIP(npbcg_dev::launch_fill(c->pb.x, n + 1, 1.0, vec_blocks, c->stream));
     CREATE_HIP(hipStreamSynchronize(c->stream));
 #undef CREATE_HIP
@@ -366,6 +436,9 @@ void npbcg_destroy(npbcg_ctx *c) {
     (void)hipFree(c->d_rowstr);
     (void)hipFree(c->d_colidx);
     (void)hipFree(c->d_a);
+    (void)hipFree(c->d_tseg);
+    (void)hipFree(c->d_tval);
+    (void)hipFree(c->d_tcol);
     (void)hipFree(c->d_vec);
     (void)hipFree(c->d_part);
     (void)hipFree(c->d_sc);

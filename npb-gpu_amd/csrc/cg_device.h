@@ -69,10 +69,17 @@ enum TicketSlot { kTicketInit = 0, kTicketSpmv = 1, kTicketZR = 2, kTicketResid 
 // in the last workgroup to finish, over all workgroups' partials.  Returns
 // true in the last workgroup, where tot[] then holds the grid totals (valid
 // in every thread of that workgroup).  part holds NV*gridDim.x doubles.
-template <int NV>
+//
+// Ordering: each workgroup stores its partial, issues an agent-scope RELEASE
+// fence (L2 write-back only) and bumps the ticket with a relaxed atomic; the
+// last workgroup issues an agent-scope ACQUIRE fence before reading the
+// partials.  A full __threadfence() in every workgroup would also invalidate
+// the XCD's L2 each time and evict the gathered vector mid-kernel.
+template <int NV, int BLOCK = kBlock>
 __device__ __forceinline__ bool grid_reduce(double (&acc)[NV], double *part,
                                             unsigned *ticket, double (&tot)[NV]) {
-    __shared__ double wsum[NV][kWavesPerBlock];
+    constexpr int kWaves = BLOCK / kWave;
+    __shared__ double wsum[NV][kWaves];
     __shared__ bool is_last;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
@@ -86,21 +93,22 @@ __device__ __forceinline__ bool grid_reduce(double (&acc)[NV], double *part,
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             double s = wsum[v][0];
-            for (int w = 1; w < kWavesPerBlock; w++) s += wsum[v][w];
+            for (int w = 1; w < kWaves; w++) s += wsum[v][w];
             part[(size_t)v * gridDim.x + blockIdx.x] = s;
         }
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
         is_last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!is_last) return false;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int nb = gridDim.x;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kBlock)
+        for (int b = threadIdx.x; b < nb; b += BLOCK)
             s += part[(size_t)v * nb + b];
         s = segment_sum<64>(s);
         if (lane == 0) wsum[v][wid] = s;
@@ -109,10 +117,11 @@ __device__ __forceinline__ bool grid_reduce(double (&acc)[NV], double *part,
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         double s = wsum[v][0];
-        for (int w = 1; w < kWavesPerBlock; w++) s += wsum[v][w];
+        for (int w = 1; w < kWaves; w++) s += wsum[v][w];
         tot[v] = s;
     }
-    if (threadIdx.x == 0) *ticket = 0u;
+    if (threadIdx.x == 0)
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 

@@ -232,6 +232,126 @@ __global__ void fill_kernel(double *__restrict__ v, int count, double value) {
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged SpMV over the column-blocked copy (tiled_layout.h).
+//
+// One 1024-thread workgroup owns a tile of T rows.  For each column block it
+// stages the W entries of the input vector the block covers into LDS with
+// coalesced 16-byte loads, then groups of SEG lanes sweep the tile's row
+// segments in that block, streaming (value, 16-bit column) pairs from HBM and
+// gathering the vector from LDS.  Each row's running sum lives in LDS and is
+// owned by one lane group, so the per-row summation order (block by block,
+// then lane-strided with a fixed DPP tree) is fixed.  The random 8-byte
+// gathers that bound the CSR kernel (one 128-byte L2 line per gather) become
+// LDS reads; the vector crosses from L2 once per tile and block.
+// ---------------------------------------------------------------------------
+
+template <int SEG, int UNR>
+__device__ __forceinline__ double seg_dot_lds(const double *__restrict__ val,
+                                              const uint16_t *__restrict__ col,
+                                              const double *vblk, int s, int e, int sl) {
+    double sum = 0.0;
+    for (int k0 = s + sl; k0 < e; k0 += SEG * UNR) {
+        double av[UNR];
+        int c[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int k = k0 + u * SEG;
+            const bool ok = k < e;
+            av[u] = ok ? val[k] : 0.0;
+            c[u] = ok ? (int)col[k] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+            if (c[u] >= 0) sum = fma(av[u], vblk[c[u]], sum);
+    }
+    return segment_sum<SEG>(sum);
+}
+
+// MODE 0: y = A v with v = p, y = q; epilogue d = p.q, alpha = rho/d.
+// MODE 1: y = A v with v = z, y = r; epilogue ||x - r||, x.z, z.z, zeta.
+template <int SEG, int UNR, int MODE>
+__global__ __launch_bounds__(kTiledBlock) void spmv_tiled_kernel(
+    const int *__restrict__ seg, const double *__restrict__ val,
+    const uint16_t *__restrict__ col, const double *__restrict__ v,
+    double *__restrict__ y, const double *__restrict__ x, int n, int T, int W, int nblk,
+    int ntile, double *__restrict__ part, Scalars *sc, double *__restrict__ zeta_hist,
+    double *__restrict__ rnorm_hist, int hist_cap, int record) {
+    extern __shared__ double2 lds_v2[];  // 16-byte aligned dynamic LDS
+    double *lds = reinterpret_cast<double *>(lds_v2);
+    double *vblk = lds;       // W entries of the input vector
+    double *rowsum = lds + W; // T running row sums
+    constexpr int G = kTiledBlock / SEG;
+    const int g = threadIdx.x / SEG, sl = threadIdx.x % SEG;
+    double acc[MODE == 0 ? 1 : 3] = {};
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const int row0 = t * T;
+        const int nrows = min(T, n - row0);
+        for (int i = threadIdx.x; i < T; i += kTiledBlock) rowsum[i] = 0.0;
+        for (int b = 0; b < nblk; b++) {
+            const int c0 = b * W;
+            const int wb = min(W, n - c0);
+            __syncthreads();  // previous block's gathers are done with vblk
+            const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
+            double2 *l2 = reinterpret_cast<double2 *>(vblk);
+#pragma unroll 4
+            for (int i = threadIdx.x; i < wb / 2; i += kTiledBlock) l2[i] = v2[i];
+            if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
+            __syncthreads();
+            const int *sp = seg + (size_t)(t * nblk + b) * (T + 1);
+            for (int r = g; r < nrows; r += G) {
+                const double s = seg_dot_lds<SEG, UNR>(val, col, vblk, sp[r], sp[r + 1], sl);
+                if (sl == 0) rowsum[r] += s;
+            }
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nrows; r += kTiledBlock) {
+            const int row = row0 + r;
+            const double yv = rowsum[r];
+            y[row] = yv;
+            if constexpr (MODE == 0) {
+                acc[0] += v[row] * yv;
+            } else {
+                const double xv = x[row], zv = v[row];
+                const double dlt = xv - yv;
+                acc[0] += dlt * dlt;
+                acc[1] += xv * zv;
+                acc[2] += zv * zv;
+            }
+        }
+        __syncthreads();  // rowsum is reset for the next tile
+    }
+    double tot[MODE == 0 ? 1 : 3];
+    if constexpr (MODE == 0) {
+        if (grid_reduce<1, kTiledBlock>(acc, part, &sc->ticket[kTicketSpmv], tot) &&
+            threadIdx.x == 0) {
+            sc->d = tot[0];
+            sc->alpha = sc->rho / tot[0];
+            sc->rho0 = sc->rho;
+        }
+    } else {
+        if (grid_reduce<3, kTiledBlock>(acc, part, &sc->ticket[kTicketResid], tot) &&
+            threadIdx.x == 0) {
+            const double rnorm = sqrt(tot[0]);
+            sc->rnorm = rnorm;
+            sc->xz = tot[1];
+            sc->zz = tot[2];
+            sc->xscale = 1.0 / sqrt(tot[2]);
+            const double zeta = sc->shift + 1.0 / tot[1];
+            sc->zeta = zeta;
+            if (record) {
+                const int it = sc->iter;
+                if (it < hist_cap) {
+                    zeta_hist[it] = zeta;
+                    rnorm_hist[it] = rnorm;
+                }
+                sc->iter = it + 1;
+                sc->pending_scale = 1;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
 
@@ -252,7 +372,44 @@ hipError_t launch_resid_t(const Fp64Problem &pb, int record, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int SEG, int MODE>
+hipError_t launch_tiled_t(const Fp64Problem &pb, int record, hipStream_t st) {
+    const TiledDev &td = pb.tiled;
+    const double *v = MODE == 0 ? pb.p : pb.z;
+    double *y = MODE == 0 ? pb.q : pb.r;
+    spmv_tiled_kernel<SEG, 2, MODE><<<td.grid, kTiledBlock, td.lds_bytes, st>>>(
+        td.seg, td.val, td.col, v, y, pb.x, pb.n, td.T, td.W, td.nblk, td.ntile, pb.part,
+        pb.sc, pb.zeta_hist, pb.rnorm_hist, pb.hist_cap, record);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_tiled(const Fp64Problem &pb, int record, hipStream_t st) {
+    switch (pb.tiled.seg_lanes) {
+    case 16: return launch_tiled_t<16, MODE>(pb, record, st);
+    case 64: return launch_tiled_t<64, MODE>(pb, record, st);
+    default: return launch_tiled_t<32, MODE>(pb, record, st);
+    }
+}
+
+template <int SEG, int MODE>
+hipError_t set_tiled_lds_t(int bytes) {
+    return hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&spmv_tiled_kernel<SEG, 2, MODE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 }  // namespace
+
+hipError_t prepare_tiled_kernels(int lds_bytes) {
+    hipError_t e = hipSuccess;
+    hipError_t r[6] = {set_tiled_lds_t<16, 0>(lds_bytes), set_tiled_lds_t<32, 0>(lds_bytes),
+                       set_tiled_lds_t<64, 0>(lds_bytes), set_tiled_lds_t<16, 1>(lds_bytes),
+                       set_tiled_lds_t<32, 1>(lds_bytes), set_tiled_lds_t<64, 1>(lds_bytes)};
+    for (hipError_t x : r)
+        if (x != hipSuccess) e = x;
+    return e;
+}
 
 int spmv_rows_per_block(int variant) {
     switch (variant) {
@@ -264,6 +421,7 @@ int spmv_rows_per_block(int variant) {
 
 hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st) {
     switch (pb.spmv_variant) {
+    case kSpmvTiled: return launch_tiled<0>(pb, 0, st);
     case kSpmvSeg16: return launch_spmv_pq_t<16, 4>(pb, st);
     case kSpmvSeg32: return launch_spmv_pq_t<32, 4>(pb, st);
     case kSpmvSeg64U2: return launch_spmv_pq_t<64, 2>(pb, st);
@@ -273,6 +431,7 @@ hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st) {
 
 hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
     switch (pb.spmv_variant) {
+    case kSpmvTiled: return launch_tiled<1>(pb, record, st);
     case kSpmvSeg16: return launch_resid_t<16, 4>(pb, record, st);
     case kSpmvSeg32: return launch_resid_t<32, 4>(pb, record, st);
     case kSpmvSeg64U2: return launch_resid_t<64, 2>(pb, record, st);

@@ -14,7 +14,22 @@ enum SpmvVariant {
     kSpmvSeg64 = 1,    // one wavefront per row, 4 loads in flight per lane
     kSpmvSeg32 = 2,    // half-wavefront per row
     kSpmvSeg16 = 3,    // quarter-wavefront per row
-    kSpmvSeg64U2 = 4   // one wavefront per row, 2 loads in flight per lane
+    kSpmvSeg64U2 = 4,  // one wavefront per row, 2 loads in flight per lane
+    kSpmvTiled = 5     // column-blocked copy, vector blocks staged in LDS
+};
+
+constexpr int kTiledBlock = 1024;  // threads per workgroup of the tiled SpMV
+
+// Device copy of the column-blocked matrix (tiled_layout.h) and its launch
+// shape.
+struct TiledDev {
+    const int *seg = nullptr;
+    const double *val = nullptr;
+    const uint16_t *col = nullptr;
+    int T = 0, W = 0, nblk = 0, ntile = 0;
+    int grid = 0;       // workgroups launched (grid-stride over tiles)
+    int seg_lanes = 32; // lanes per row segment: 16, 32 or 64
+    int lds_bytes = 0;  // dynamic LDS: (W + T) doubles
 };
 
 // Device pointers and launch shapes of one FP64 problem.
@@ -32,6 +47,7 @@ struct Fp64Problem {
     int vec_blocks;             // workgroups of the element-wise kernels
     int spmv_blocks;            // workgroups of the SpMV kernels
     int spmv_variant;
+    TiledDev tiled;
 };
 
 // Optional HIP events bracketing the first `count` SpMV launches of a
@@ -43,6 +59,9 @@ struct SpmvProbe {
 };
 
 int spmv_rows_per_block(int variant);
+
+// Raises the dynamic-LDS limit of the tiled SpMV kernels to lds_bytes.
+hipError_t prepare_tiled_kernels(int lds_bytes);
 
 hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st);
 hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st);

@@ -29,7 +29,7 @@ ERR_NOCONV = -8
 
 ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
 VEC_X, VEC_Z, VEC_P, VEC_Q, VEC_R = 0, 1, 2, 3, 4
-SPMV_AUTO, SPMV_SEG64, SPMV_SEG32, SPMV_SEG16, SPMV_SEG64_U2 = 0, 1, 2, 3, 4
+SPMV_AUTO, SPMV_SEG64, SPMV_SEG32, SPMV_SEG16, SPMV_SEG64_U2, SPMV_TILED = 0, 1, 2, 3, 4, 5
 
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -45,7 +45,8 @@ class NpbcgError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("cgitmax", ctypes.c_int),
                 ("engine", ctypes.c_int), ("spmv_variant", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("reserved", ctypes.c_int * 11)]
+                ("use_graph", ctypes.c_int), ("tile_rows", ctypes.c_int),
+                ("tile_width", ctypes.c_int), ("reserved", ctypes.c_int * 9)]
 
 
 class Timing(ctypes.Structure):

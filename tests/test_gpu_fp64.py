@@ -99,7 +99,7 @@ def test_power_large_matches_reference(gpu, golden, name):
         assert abs(zeta[-1] - zref) / zref <= oracle.NPB_EPSILON
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("name", ["cfg1_n1400_seed12345", "A_npb"])
 def test_spmv_variants(gpu, golden, name, variant):
     c = golden[name]
@@ -116,6 +116,35 @@ def test_spmv_variants(gpu, golden, name, variant):
                f"{name} spmv variant {variant}")
     _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"][:3]],
                 f"{name} variant {variant}")
+
+
+@pytest.mark.parametrize("tile_rows,tile_width", [(1, 2), (7, 130), (64, 511), (1400, 1400),
+                                                  (333, 4096)])
+def test_tiled_layout_shapes(gpu, golden, tile_rows, tile_width):
+    # Many column blocks, an odd-width last block, partial last tiles, one
+    # row per tile: every shape must give the oracle's answer.
+    c = golden["cfg1_n1400_seed12345"]
+    m = _gen(gpu, c)
+    n = m.n
+    x = np.ones(n + 2)
+    x[:n] += np.linspace(-0.3, 0.3, n)
+    with gpu.Context(m, spmv_variant=5, tile_rows=tile_rows, tile_width=tile_width) as ctx:
+        z, p, q, r, rn = ctx.conj_grad(x)
+        zeta, _ = ctx.power(4, c["shift"])
+    oz, op, oq, orr, orn = oracle.conj_grad(_ocsr(m), x)
+    for v, ref, lab in [(z, oz, "z"), (q, oq, "q"), (r, orr, "r")]:
+        _vec_close(v[:n], ref[:n], 1.3, f"tiled {tile_rows}x{tile_width} {lab}")
+    assert abs(rn - orn) <= 1e-12 * np.linalg.norm(x[:n])
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"][:4]], "tiled power")
+
+
+@pytest.mark.parametrize("variant", [1, 5])
+def test_class_b_variants_match_reference(gpu, golden, variant):
+    c = golden["B_npb"]
+    m = _gen(gpu, c)
+    with gpu.Context(m, spmv_variant=variant) as ctx:
+        zeta, _ = ctx.power(c["niter"], c["shift"])
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"]], f"B variant {variant}")
 
 
 def test_power_is_deterministic_and_graph_matches_direct(gpu, golden):
