@@ -218,6 +218,7 @@ def main():
         "zeta_final": float(zeta[-1]),
         "zeta_max_rel_err_vs_reference": zeta_err,
         "generate_seconds": round(t_gen, 2),
+        "kernel_config": ctx.describe(),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

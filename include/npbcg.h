@@ -145,9 +145,15 @@ int npbcg_fill_x(npbcg_ctx *ctx, double value);
 int npbcg_set_spmv_sampling(npbcg_ctx *ctx, int every_nth_call);
 int npbcg_get_timing(const npbcg_ctx *ctx, npbcg_timing *out);
 
-/* One q = A p product with the context's kernel on its device vectors,
- * repeated reps times; *avg_ms gets the mean duration from HIP events.
- * Used to measure SpMV bandwidth in isolation. */
+/* The context's kernel configuration as a one-line JSON object (SpMV variant,
+ * block/tile shape, workgroup counts, stored entries) written to buf
+ * (NUL-terminated, truncated to len).  Returns the full length. */
+int npbcg_describe(const npbcg_ctx *ctx, char *buf, int len);
+
+/* One complete q = A p product with the context's kernels on its device
+ * vectors (for the LDS-staged layout: the SpMV and the combine of its
+ * per-block row sums), repeated reps times; *avg_ms gets the mean duration
+ * from HIP events.  Used to measure SpMV bandwidth in isolation. */
 int npbcg_spmv_bench(npbcg_ctx *ctx, int reps, double *avg_ms);
 
 /* ---- reference-shaped shim ------------------------------------------------

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -57,20 +58,25 @@ struct npbcg_ctx {
     int *d_rowstr = nullptr;
     int *d_colidx = nullptr;
     double *d_a = nullptr;
-    int *d_tseg = nullptr;         // column-blocked copy (tiled SpMV)
-    double *d_tval = nullptr;
-    uint16_t *d_tcol = nullptr;
+    int *d_sell_idx = nullptr;     // column-blocked SELL copy: wg_slice, slice_off, slice_row
+    double *d_sell_val = nullptr;
+    uint16_t *d_sell_col = nullptr;
+    double *d_rows_part = nullptr; // nblk * n per-block row sums
     bool cols_sorted = true;
+    long long sell_stored = 0;     // SELL entries, padding included
     double *d_vec = nullptr;   // x, z, p, q, r back to back, stride vstride
     size_t vstride = 0;
-    double *d_part = nullptr;
+    double *d_part = nullptr;  // all partial-sum buffers, carved by refresh_problem
+    int part_vec = 0, part_spmv = 0;
     Scalars *d_sc = nullptr;
     double *d_hist = nullptr;  // zeta then rnorm, hist_cap each
     int hist_cap = 0;
     Fp64Problem pb{};
 
-    hipGraphExec_t outer_graph = nullptr;
-    hipGraph_t outer_graph_src = nullptr;
+    // Captured outer iterations: [0] first (x used as is), [1] later ones
+    // (x = z/||z|| of the previous iteration applied first).
+    hipGraphExec_t outer_graph[2] = {nullptr, nullptr};
+    hipGraph_t outer_graph_src[2] = {nullptr, nullptr};
 
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> probe_start, probe_stop;
@@ -114,7 +120,14 @@ void refresh_problem(npbcg_ctx *c) {
     pb.p = vec_ptr(c, NPBCG_VEC_P);
     pb.q = vec_ptr(c, NPBCG_VEC_Q);
     pb.r = vec_ptr(c, NPBCG_VEC_R);
-    pb.part = c->d_part;
+    double *pp = c->d_part;
+    const size_t nv = c->part_vec, ns = c->part_spmv;
+    pb.part.xx = pp;
+    pb.part.rr[0] = pp + nv;
+    pb.part.rr[1] = pp + 2 * nv;
+    pb.part.nrm = pp + 3 * nv;           // 2 * nv
+    pb.part.pq = pp + 5 * nv;            // ns
+    pb.part.res = pp + 5 * nv + ns;      // 3 * ns
     pb.sc = c->d_sc;
     pb.zeta_hist = c->d_hist;
     pb.rnorm_hist = c->d_hist ? c->d_hist + c->hist_cap : nullptr;
@@ -122,10 +135,12 @@ void refresh_problem(npbcg_ctx *c) {
 }
 
 void drop_graph(npbcg_ctx *c) {
-    if (c->outer_graph) (void)hipGraphExecDestroy(c->outer_graph);
-    if (c->outer_graph_src) (void)hipGraphDestroy(c->outer_graph_src);
-    c->outer_graph = nullptr;
-    c->outer_graph_src = nullptr;
+    for (int i = 0; i < 2; i++) {
+        if (c->outer_graph[i]) (void)hipGraphExecDestroy(c->outer_graph[i]);
+        if (c->outer_graph_src[i]) (void)hipGraphDestroy(c->outer_graph_src[i]);
+        c->outer_graph[i] = nullptr;
+        c->outer_graph_src[i] = nullptr;
+    }
 }
 
 int check_csr(int n, const int *rowstr, const int *colidx, long long *nnz_out,
@@ -189,10 +204,10 @@ int ensure_history(npbcg_ctx *c, int niter) {
     return NPBCG_OK;
 }
 
-int ensure_graph(npbcg_ctx *c) {
-    if (c->outer_graph) return NPBCG_OK;
+int capture_outer(npbcg_ctx *c, int scale_x, hipGraph_t *g_out, hipGraphExec_t *ex_out) {
     NPBCG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 1, c->stream, nullptr);
+    hipError_t e =
+        npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, scale_x, 1, c->stream, nullptr);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -208,8 +223,17 @@ int ensure_graph(npbcg_ctx *c) {
         return set_error(NPBCG_ERR_DEVICE,
                          std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
     }
-    c->outer_graph_src = g;
-    c->outer_graph = ex;
+    *g_out = g;
+    *ex_out = ex;
+    return NPBCG_OK;
+}
+
+int ensure_graph(npbcg_ctx *c) {
+    for (int i = 0; i < 2; i++) {
+        if (c->outer_graph[i]) continue;
+        int rc = capture_outer(c, i, &c->outer_graph_src[i], &c->outer_graph[i]);
+        if (rc != NPBCG_OK) return rc;
+    }
     return NPBCG_OK;
 }
 
@@ -326,40 +350,56 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     const int rpb = npbcg_dev::spmv_rows_per_block(variant);
     const int vec_blocks = std::max(1, std::min((n + 1 + npbcg_dev::kBlock - 1) /
                                                     npbcg_dev::kBlock, 8 * cus));
-    const int spmv_blocks = std::max(1, std::min((n + rpb - 1) / rpb, 8 * cus));
+    const int spmv_blocks = std::max(1, std::min((n + rpb - 1) / rpb, 4 * cus));
 
-    // Column-blocked copy for the LDS-staged SpMV: one 1024-thread workgroup
-    // per tile, the tiles spread evenly over the CUs, W vector entries per
-    // LDS block (W * 8 bytes + T running sums must fit the CU's LDS).
-    npbcg_detail::TiledMatrix<double> tm;
-    npbcg_dev::TiledDev td;
+    // Column-blocked SELL copy for the LDS-staged SpMV.  One column block of
+    // the vector must fit the CU's LDS: the widest blocks (fewest partial
+    // sums) take one 1024-thread workgroup per CU; blocks that fit twice use
+    // 512-thread workgroups, two per CU.  Row ranges spread the
+    // nblk * nrange workgroups evenly over the CUs.
+    npbcg_detail::SellBlocks<double> sm;
+    npbcg_dev::SellDev sd;
     if (variant == npbcg_dev::kSpmvTiled) {
         int lds_cap = 0;
         if (hipDeviceGetAttribute(&lds_cap, hipDeviceAttributeSharedMemPerBlockOptin,
                                   c->device) != hipSuccess || lds_cap <= 0)
             lds_cap = 64 * 1024;
         const int lds_budget = lds_cap - 2048;  // static LDS of the reduction epilogue
-        int W = c->opts.tile_width > 0 ? c->opts.tile_width
-                                       : std::min(16384, (lds_budget / 8) * 3 / 4);
-        W = std::max(2, W & ~1);
-        if (W > n) W = (n + 1) & ~1;
-        const int per_cu = std::max(1, (int)((n + (long long)cus * 2048 - 1) /
-                                             ((long long)cus * 2048)));
-        const int T = c->opts.tile_rows > 0
-                          ? c->opts.tile_rows
-                          : std::max(1, (n + cus * per_cu - 1) / (cus * per_cu));
-        if ((W + T) * 8 > lds_budget ||
-            !npbcg_detail::build_tiled(n, rowstr, colidx, a, T, W, &tm)) {
-            variant = pick_variant(npbcg_dev::kSpmvSeg64, n, nnz, sorted);
+        int W;
+        if (c->opts.tile_width > 0) {
+            W = c->opts.tile_width;
         } else {
-            td.T = tm.T;
-            td.W = tm.W;
-            td.nblk = tm.nblk;
-            td.ntile = tm.ntile;
-            td.grid = tm.ntile;
-            td.lds_bytes = (tm.W + tm.T) * 8;
-            const double seg_len = (double)nnz / ((double)n * tm.nblk);
-            td.seg_lanes = seg_len >= 48.0 ? 64 : (seg_len >= 20.0 ? 32 : 16);
+            const int wmax = std::min(65536, lds_budget / 8);
+            const int nblk = (n + wmax - 1) / wmax;
+            W = (n + nblk - 1) / nblk;
+        }
+        W = std::max(2, (W + 1) & ~1);
+        if (W > n) W = (n + 1) & ~1;
+        const int block = (W * 8 <= lds_budget / 2 - 1024) ? 512 : 1024;
+        const int nblk = (n + W - 1) / W;
+        int R;
+        if (c->opts.tile_rows > 0) {
+            R = c->opts.tile_rows;
+        } else {
+            const int slots = cus * (block == 512 ? 2 : 1);
+            const int nrange = std::max(1, slots / nblk);
+            R = (n + nrange - 1) / nrange;
+        }
+        if (W * 8 > lds_budget || !sorted ||
+            !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, &sm)) {
+            variant = pick_variant(npbcg_dev::kSpmvSeg64, n, nnz, false);
+        } else {
+            sd.W = sm.W;
+            sd.nblk = sm.nblk;
+            sd.R = sm.R;
+            sd.nrange = sm.nrange;
+            sd.grid = sm.nblk * sm.nrange;
+            sd.block = block;
+            int max_slices = 0;
+            for (size_t w = 0; w + 1 < sm.wg_slice.size(); w++)
+                max_slices = std::max(max_slices, sm.wg_slice[w + 1] - sm.wg_slice[w]);
+            sd.lds_bytes = (sm.W + max_slices) * 8;
+            if (sd.lds_bytes > lds_budget) variant = pick_variant(npbcg_dev::kSpmvSeg64, n, nnz, false);
         }
     }
 
@@ -368,25 +408,39 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     CREATE_HIP(hipMalloc(&c->d_colidx, sizeof(int) * std::max<long long>(1, nnz)));
     CREATE_HIP(hipMalloc(&c->d_a, sizeof(double) * std::max<long long>(1, nnz)));
     CREATE_HIP(hipMalloc(&c->d_vec, sizeof(double) * 5 * c->vstride));
+    c->part_vec = vec_blocks;
+    c->part_spmv = variant == npbcg_dev::kSpmvTiled ? std::max(sd.grid, vec_blocks)
+                                                    : spmv_blocks;
     CREATE_HIP(hipMalloc(&c->d_part,
-                         sizeof(double) * 3 *
-                             (size_t)std::max({vec_blocks, spmv_blocks, td.grid})));
+                         sizeof(double) * (5 * (size_t)c->part_vec + 4 * (size_t)c->part_spmv)));
     if (variant == npbcg_dev::kSpmvTiled) {
-        CREATE_HIP(npbcg_dev::prepare_tiled_kernels(td.lds_bytes));
-        CREATE_HIP(hipMalloc(&c->d_tseg, sizeof(int) * tm.seg.size()));
-        CREATE_HIP(hipMalloc(&c->d_tval, sizeof(double) * std::max<size_t>(1, tm.val.size())));
-        CREATE_HIP(hipMalloc(&c->d_tcol, sizeof(uint16_t) * std::max<size_t>(1, tm.col.size())));
-        CREATE_HIP(hipMemcpy(c->d_tseg, tm.seg.data(), sizeof(int) * tm.seg.size(),
+        CREATE_HIP(npbcg_dev::prepare_sell_kernels(sd.lds_bytes));
+        const size_t nwg1 = sm.wg_slice.size(), nsl1 = sm.slice_off.size();
+        const size_t idx = nwg1 + nsl1 + sm.slice_row.size();
+        CREATE_HIP(hipMalloc(&c->d_sell_idx, sizeof(int) * idx));
+        CREATE_HIP(hipMalloc(&c->d_sell_val, sizeof(double) * std::max<size_t>(1, sm.val.size())));
+        CREATE_HIP(hipMalloc(&c->d_sell_col,
+                             sizeof(uint16_t) * std::max<size_t>(1, sm.col.size())));
+        CREATE_HIP(hipMalloc(&c->d_rows_part, sizeof(double) * (size_t)sm.nblk * n));
+        CREATE_HIP(hipMemcpy(c->d_sell_idx, sm.wg_slice.data(), sizeof(int) * nwg1,
                              hipMemcpyHostToDevice));
-        if (!tm.val.empty()) {
-            CREATE_HIP(hipMemcpy(c->d_tval, tm.val.data(), sizeof(double) * tm.val.size(),
+        CREATE_HIP(hipMemcpy(c->d_sell_idx + nwg1, sm.slice_off.data(), sizeof(int) * nsl1,
+                             hipMemcpyHostToDevice));
+        CREATE_HIP(hipMemcpy(c->d_sell_idx + nwg1 + nsl1, sm.slice_row.data(),
+                             sizeof(int) * sm.slice_row.size(), hipMemcpyHostToDevice));
+        if (!sm.val.empty()) {
+            CREATE_HIP(hipMemcpy(c->d_sell_val, sm.val.data(), sizeof(double) * sm.val.size(),
                                  hipMemcpyHostToDevice));
-            CREATE_HIP(hipMemcpy(c->d_tcol, tm.col.data(), sizeof(uint16_t) * tm.col.size(),
-                                 hipMemcpyHostToDevice));
+            CREATE_HIP(hipMemcpy(c->d_sell_col, sm.col.data(),
+                                 sizeof(uint16_t) * sm.col.size(), hipMemcpyHostToDevice));
         }
-        td.seg = c->d_tseg;
-        td.val = c->d_tval;
-        td.col = c->d_tcol;
+        sd.wg_slice = c->d_sell_idx;
+        sd.slice_off = c->d_sell_idx + nwg1;
+        sd.slice_row = c->d_sell_idx + nwg1 + nsl1;
+        sd.val = c->d_sell_val;
+        sd.col = c->d_sell_col;
+        sd.rows_part = c->d_rows_part;
+        c->sell_stored = sm.stored;
     }
     CREATE_HIP(hipMalloc(&c->d_sc, npbcg_dev::scalars_bytes()));
     CREATE_HIP(hipHostMalloc(&c->h_stage,
@@ -407,7 +461,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     c->pb.vec_blocks = vec_blocks;
     c->pb.spmv_blocks = spmv_blocks;
     c->pb.spmv_variant = variant;
-    c->pb.tiled = td;
+    c->pb.sell = sd;
     CREATE_HIP(npbcg_dev::launch_fill(c->pb.x, n + 1, 1.0, vec_blocks, c->stream));
     CREATE_HIP(hipStreamSynchronize(c->stream));
 #undef CREATE_HIP
@@ -436,9 +490,10 @@ void npbcg_destroy(npbcg_ctx *c) {
     (void)hipFree(c->d_rowstr);
     (void)hipFree(c->d_colidx);
     (void)hipFree(c->d_a);
-    (void)hipFree(c->d_tseg);
-    (void)hipFree(c->d_tval);
-    (void)hipFree(c->d_tcol);
+    (void)hipFree(c->d_sell_idx);
+    (void)hipFree(c->d_sell_val);
+    (void)hipFree(c->d_sell_col);
+    (void)hipFree(c->d_rows_part);
     (void)hipFree(c->d_vec);
     (void)hipFree(c->d_part);
     (void)hipFree(c->d_sc);
@@ -451,9 +506,8 @@ void npbcg_destroy(npbcg_ctx *c) {
 int npbcg_conj_grad_device(npbcg_ctx *c, double *rnorm) {
     if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_conj_grad_device: NULL context");
     NPBCG_HIP(hipSetDevice(c->device));
-    NPBCG_HIP(npbcg_dev::launch_clear_pending(c->pb, c->stream));
     NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
-    NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 0, c->stream, nullptr));
+    NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 0, 0, c->stream, nullptr));
     NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
     Scalars s;
     int rc = read_scalars(c, &s);
@@ -463,7 +517,7 @@ int npbcg_conj_grad_device(npbcg_ctx *c, double *rnorm) {
     c->timing.total_ms = ms;
     c->timing.spmv_samples = 0;
     c->timing.spmv_avg_ms = 0.0;
-    c->timing.launches = 3 * c->cgitmax + 2;
+    c->timing.launches = 3 * c->cgitmax + 3;
     if (rnorm) *rnorm = s.rnorm;
     clear_error();
     return NPBCG_OK;
@@ -542,19 +596,21 @@ int npbcg_power(npbcg_ctx *c, int niter, double shift, double *zeta_out,
     NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
     int launches = 0;
     for (int it = 0; it < niter; it++) {
+        const int scale = it > 0 ? 1 : 0;
         if (sample && it == 0) {
-            NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 1, c->stream, &probe));
-            launches += 3 * c->cgitmax + 2;
+            NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, scale, 1, c->stream,
+                                                   &probe));
+            launches += 3 * c->cgitmax + 3;
         } else if (use_graph) {
-            NPBCG_HIP(hipGraphLaunch(c->outer_graph, c->stream));
+            NPBCG_HIP(hipGraphLaunch(c->outer_graph[scale], c->stream));
             launches += 1;
         } else {
-            NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 1, c->stream, nullptr));
-            launches += 3 * c->cgitmax + 2;
+            NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, scale, 1, c->stream,
+                                                   nullptr));
+            launches += 3 * c->cgitmax + 3;
         }
     }
-    NPBCG_HIP(npbcg_dev::launch_scale_x(c->pb, 0.0, 1, c->stream));
-    NPBCG_HIP(npbcg_dev::launch_clear_pending(c->pb, c->stream));
+    if (niter > 0) NPBCG_HIP(npbcg_dev::launch_scale_x(c->pb, 0.0, 1, c->stream));
     NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
     NPBCG_HIP(hipStreamSynchronize(c->stream));
     if (niter > 0 && zeta_out)
@@ -613,7 +669,6 @@ int npbcg_fill_x(npbcg_ctx *c, double value) {
     if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_fill_x: NULL context");
     NPBCG_HIP(hipSetDevice(c->device));
     NPBCG_HIP(npbcg_dev::launch_fill(c->pb.x, c->n + 1, value, c->pb.vec_blocks, c->stream));
-    NPBCG_HIP(npbcg_dev::launch_clear_pending(c->pb, c->stream));
     NPBCG_HIP(hipStreamSynchronize(c->stream));
     clear_error();
     return NPBCG_OK;
@@ -633,13 +688,35 @@ int npbcg_get_timing(const npbcg_ctx *c, npbcg_timing *out) {
     return NPBCG_OK;
 }
 
+int npbcg_describe(const npbcg_ctx *c, char *buf, int len) {
+    if (!c || (!buf && len > 0) || len < 0)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_describe: invalid argument");
+    const npbcg_dev::SellDev &sd = c->pb.sell;
+    char tmp[512];
+    int k = std::snprintf(
+        tmp, sizeof tmp,
+        "{\"n\": %d, \"nnz\": %lld, \"spmv_variant\": %d, \"vec_blocks\": %d, "
+        "\"csr_blocks\": %d, \"block_width\": %d, \"col_blocks\": %d, "
+        "\"range_rows\": %d, \"row_ranges\": %d, \"sell_workgroups\": %d, "
+        "\"sell_threads\": %d, \"sell_stored\": %lld, \"cgitmax\": %d, "
+        "\"use_graph\": %d}",
+        c->n, c->nnz, c->pb.spmv_variant, c->pb.vec_blocks, c->pb.spmv_blocks, sd.W,
+        sd.nblk, sd.R, sd.nrange, sd.grid, sd.block, c->sell_stored, c->cgitmax,
+        c->opts.use_graph);
+    if (len > 0) {
+        std::strncpy(buf, tmp, (size_t)len - 1);
+        buf[len - 1] = '\0';
+    }
+    return k;
+}
+
 int npbcg_spmv_bench(npbcg_ctx *c, int reps, double *avg_ms) {
     if (!c || reps < 1 || !avg_ms)
         return set_error(NPBCG_ERR_INVALID, "npbcg_spmv_bench: invalid argument");
     NPBCG_HIP(hipSetDevice(c->device));
-    NPBCG_HIP(npbcg_dev::launch_spmv_pq(c->pb, c->stream));  // warm
+    NPBCG_HIP(npbcg_dev::launch_spmv_q(c->pb, c->stream));  // warm
     NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
-    for (int i = 0; i < reps; i++) NPBCG_HIP(npbcg_dev::launch_spmv_pq(c->pb, c->stream));
+    for (int i = 0; i < reps; i++) NPBCG_HIP(npbcg_dev::launch_spmv_q(c->pb, c->stream));
     NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
     NPBCG_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;

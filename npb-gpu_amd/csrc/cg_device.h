@@ -42,9 +42,11 @@ __device__ __forceinline__ double segment_sum(double v) {
     return v;
 }
 
-// Device-resident CG scalars of one context.  Written by the last workgroup
-// of the kernel that completes each reduction and read by later kernels, so
-// no scalar crosses to the host inside the solver loops.
+// Device-resident CG scalars of one context.  Kernels never pass values to
+// other workgroups of the same launch: each workgroup stores a partial sum and
+// the next kernel in the stream reduces the partials (every workgroup in the
+// same fixed order, so all obtain the same bits).  Scalars that outlive a
+// launch (written by workgroup 0) are read only by later launches.
 struct Scalars {
     double rho;      // r.r of the current residual
     double rho0;     // previous rho
@@ -58,71 +60,44 @@ struct Scalars {
     double zeta;     // shift + 1/(x.z)
     double shift;
     int iter;        // outer iterations recorded so far in the history
-    int pending_scale;  // 1: x must become xscale * z before the next conj_grad
-    unsigned ticket[8]; // last-workgroup counters, one per reduction site
+    int pad;
 };
 
-enum TicketSlot { kTicketInit = 0, kTicketSpmv = 1, kTicketZR = 2, kTicketResid = 3,
-                  kTicketNorms = 4 };
-
-// Reduce NV per-thread accumulators over the workgroup (fixed order) and,
-// in the last workgroup to finish, over all workgroups' partials.  Returns
-// true in the last workgroup, where tot[] then holds the grid totals (valid
-// in every thread of that workgroup).  part holds NV*gridDim.x doubles.
-//
-// Ordering: each workgroup stores its partial, issues an agent-scope RELEASE
-// fence (L2 write-back only) and bumps the ticket with a relaxed atomic; the
-// last workgroup issues an agent-scope ACQUIRE fence before reading the
-// partials.  A full __threadfence() in every workgroup would also invalidate
-// the XCD's L2 each time and evict the gathered vector mid-kernel.
-template <int NV, int BLOCK = kBlock>
-__device__ __forceinline__ bool grid_reduce(double (&acc)[NV], double *part,
-                                            unsigned *ticket, double (&tot)[NV]) {
+// Sum of this workgroup's per-thread values, written to part[blockIdx.x]
+// (one store per workgroup; fixed order: DPP tree per wave, waves in order).
+template <int BLOCK>
+__device__ __forceinline__ void store_block_partial(double acc, double *part) {
     constexpr int kWaves = BLOCK / kWave;
-    __shared__ double wsum[NV][kWaves];
-    __shared__ bool is_last;
+    __shared__ double wsum[kWaves];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        double s = segment_sum<64>(acc[v]);
-        if (lane == 0) wsum[v][wid] = s;
-    }
+    const double s = segment_sum<64>(acc);
+    if (lane == 0) wsum[wid] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-            double s = wsum[v][0];
-            for (int w = 1; w < kWaves; w++) s += wsum[v][w];
-            part[(size_t)v * gridDim.x + blockIdx.x] = s;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x - 1);
+        double t = wsum[0];
+        for (int w = 1; w < kWaves; w++) t += wsum[w];
+        part[blockIdx.x] = t;
     }
+}
+
+// Sum of part[0..count) by the whole workgroup, in a fixed order that depends
+// only on count and BLOCK; every thread receives the result.
+template <int BLOCK>
+__device__ __forceinline__ double reduce_partials(const double *__restrict__ part, int count) {
+    constexpr int kWaves = BLOCK / kWave;
+    __shared__ double wsum[kWaves];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count; i += BLOCK) s += part[i];
+    s = segment_sum<64>(s);
+    if (lane == 0) wsum[wid] = s;
     __syncthreads();
-    if (!is_last) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const int nb = gridDim.x;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += BLOCK)
-            s += part[(size_t)v * nb + b];
-        s = segment_sum<64>(s);
-        if (lane == 0) wsum[v][wid] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        double s = wsum[v][0];
-        for (int w = 1; w < kWaves; w++) s += wsum[v][w];
-        tot[v] = s;
-    }
-    if (threadIdx.x == 0)
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+    double t = wsum[0];
+    for (int w = 1; w < kWaves; w++) t += wsum[w];
+    __syncthreads();  // wsum may be reused by the caller's next reduction
+    return t;
 }
 
 }  // namespace npbcg_dev

@@ -3,23 +3,30 @@
 // One inner CG iteration of the reference (CUDA/CG/cg.cu:814-837) is six
 // launches and two blocking device-to-host copies there (k3 SpMV, k4 p.q,
 // k5 two axpys, k6 r.r, k7 p update, with alpha/beta formed on the host).
-// Here it is three launches with every scalar kept on the device:
-//   spmv_pq    q = A p (one wavefront or sub-wavefront per row) and p.q;
-//              the last workgroup forms alpha = rho/d            (k3+k4)
-//   update_zr  z += alpha p, r -= alpha q and r.r; the last workgroup
-//              forms beta = rho/rho0                               (k5+k6)
-//   update_p   p = r + beta p                                      (k7)
+// Here it is three launches and nothing returns to the host:
+//   spmv       q = A p, per-workgroup partials of p.q                 (k3+k4)
+//   update_zr  reduces rho and p.q partials -> alpha; z += alpha p,
+//              r -= alpha q; partials of the new r.r                  (k5+k6)
+//   update_p   reduces old and new r.r -> beta; p = r + beta p        (k7)
 // Around the inner loop:
-//   cg_init    q = z = 0, r = p = x through index n, rho = x.x; applies a
-//              pending x = z/||z|| of the previous outer iteration (k11+k1+k2)
-//   resid      r = A z, ||x - r||, x.z, z.z and zeta            (k8+k9+k10)
-// Reductions are two-level and in a fixed order (workgroup partials, then the
-// last workgroup to finish sums them in index order), so results are
-// reproducible run to run.
-#include "cg_kernels.h"
+//   cg_init    optionally x = xscale z (the previous outer step's
+//              normalisation); q = z = 0, r = p = x through index n;
+//              partials of x.x                                   (k11+k1+k2)
+//   resid      r = A z with partials of ||x - r||^2, x.z, z.z, then a
+//              one-workgroup finish forming rnorm, zeta, xscale (k8+k9+k10)
+//
+// No kernel hands values to other workgroups of its own launch, so there are
+// no grid-wide fences or atomics: each workgroup stores one partial and the
+// next launch reduces all partials in a fixed order (every consumer
+// workgroup computes the same bits).  Kernel boundaries give the ordering.
 #include "cg_device.h"
+#include "cg_kernels.h"
 
 namespace npbcg_dev {
+
+// ---------------------------------------------------------------------------
+// CSR SpMV (one group of SEG lanes per row)
+// ---------------------------------------------------------------------------
 
 // Dot product of one CSR row with v, by a group of SEG lanes.  Each lane
 // walks the row with stride SEG; UNR independent (colidx, a) pairs are loaded
@@ -47,75 +54,238 @@ __device__ __forceinline__ double row_dot(const int *__restrict__ colidx,
     return segment_sum<SEG>(sum);
 }
 
-// q = A p and d = p.q; last workgroup: alpha = rho/d, rho0 = rho.
-template <int SEG, int UNR>
-__global__ __launch_bounds__(kBlock) void spmv_pq_kernel(
+// MODE 0: y = q = A p, partial p.q.
+// MODE 1: y = r = A z, partials ||x - r||^2, x.z, z.z (stride gridDim.x).
+template <int SEG, int UNR, int MODE>
+__global__ __launch_bounds__(kBlock) void spmv_csr_kernel(
     const int *__restrict__ rowstr, const int *__restrict__ colidx,
-    const double *__restrict__ a, const double *__restrict__ p,
-    double *__restrict__ q, int n, double *__restrict__ part, Scalars *sc) {
+    const double *__restrict__ a, const double *__restrict__ v,
+    double *__restrict__ y, const double *__restrict__ x, int n,
+    double *__restrict__ part) {
     constexpr int kGroups = kBlock / SEG;
     const int sl = threadIdx.x % SEG;
-    const int g = blockIdx.x * kGroups + threadIdx.x / SEG;
+    // With a whole wavefront per row the row index is wave-uniform; saying so
+    // (readfirstlane) keeps the row loop's exit a uniform branch around the
+    // cross-lane reduction.
+    const int group = SEG == kWave ? __builtin_amdgcn_readfirstlane(threadIdx.x / SEG)
+                                   : (int)(threadIdx.x / SEG);
+    const int g = blockIdx.x * kGroups + group;
     const int ng = gridDim.x * kGroups;
-    double acc[1] = {0.0};
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     for (int row = g; row < n; row += ng) {
         const int s = rowstr[row], e = rowstr[row + 1];
-        const double sum = row_dot<SEG, UNR>(colidx, a, p, s, e, sl);
+        const double sum = row_dot<SEG, UNR>(colidx, a, v, s, e, sl);
         if (sl == 0) {
-            q[row] = sum;
-            acc[0] += p[row] * sum;
+            y[row] = sum;
+            if constexpr (MODE == 0) {
+                acc0 += v[row] * sum;
+            } else {
+                const double xv = x[row], zv = v[row];
+                const double dlt = xv - sum;
+                acc0 += dlt * dlt;
+                acc1 += xv * zv;
+                acc2 += zv * zv;
+            }
         }
     }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &sc->ticket[kTicketSpmv], tot) && threadIdx.x == 0) {
-        sc->d = tot[0];
-        sc->alpha = sc->rho / tot[0];
-        sc->rho0 = sc->rho;
+    store_block_partial<kBlock>(acc0, part);
+    if constexpr (MODE == 1) {
+        __syncthreads();
+        store_block_partial<kBlock>(acc1, part + gridDim.x);
+        __syncthreads();
+        store_block_partial<kBlock>(acc2, part + 2 * gridDim.x);
     }
 }
 
-// z += alpha p; r -= alpha q; rho = r.r; last workgroup: beta = rho/rho0.
-__global__ __launch_bounds__(kBlock) void update_zr_kernel(
-    const double *__restrict__ p, const double *__restrict__ q,
-    double *__restrict__ z, double *__restrict__ r, int n,
-    double *__restrict__ part, Scalars *sc) {
-    const double alpha = sc->alpha;
-    double acc[1] = {0.0};
+// ---------------------------------------------------------------------------
+// LDS-staged SpMV over the column-blocked SELL copy (tiled_layout.h).
+//
+// Workgroup wg owns column block b = wg / nrange of row range wg % nrange.
+// It stages the block's W entries of the input vector into LDS once, with
+// coalesced 16-byte loads, then each wavefront streams 64-row slices: lane l
+// accumulates its own row down the slice (value and 16-bit column loads are
+// contiguous across the wavefront; UNR entries are in flight per lane), so no
+// cross-lane reduction is needed.  The row's sum over this block goes to
+// part[b * n + row]; update_zr (or resid_combine) adds the blocks in order
+// b = 0, 1, ... .  The random 8-byte gathers that bound the CSR kernel (one
+// 128-byte L2 line per gather) become LDS reads, and the vector crosses from
+// L2 into LDS once per workgroup.  For q = A p the workgroup also forms its
+// partial of p.q = sum_b sum_rows p[row] * part[b][row].
+// ---------------------------------------------------------------------------
+
+template <int BLOCK, int UNR, int MODE>
+__global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
+    const int *__restrict__ wg_slice, const int *__restrict__ slice_off,
+    const int *__restrict__ slice_row, const double *__restrict__ val,
+    const uint16_t *__restrict__ col, const double *__restrict__ v,
+    double *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W,
+    int nrange) {
+    // Dynamic LDS: W vector entries, then one p.q slot per slice of this
+    // workgroup (the p.q partial is summed in slice order).
+    extern __shared__ double2 lds_v2[];  // 16-byte aligned
+    double *vblk = reinterpret_cast<double *>(lds_v2);
+    double *slice_dot = vblk + W;
+    constexpr int kWaves = BLOCK / kWave;
+    const int wg = blockIdx.x;
+    const int b = wg / nrange;
+    const int c0 = b * W;
+    const int wb = min(W, n - c0);
+    {
+        const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
+#pragma unroll 4
+        for (int i = threadIdx.x; i < wb / 2; i += BLOCK) lds_v2[i] = v2[i];
+        if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    // Wave-uniform indices (readfirstlane): every branch below that depends on
+    // them is a scalar branch, so the wavefront stays converged around the
+    // cross-lane reduction.
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    double *part_b = rows_part + (size_t)b * n;
+    const int s0 = wg_slice[wg];
+    const int nsl = wg_slice[wg + 1] - s0;
+    const int rounds = (nsl + kWaves - 1) / kWaves;
+    for (int rd = 0; rd < rounds; rd++) {
+        // Slices are sorted widest first; alternating the direction of each
+        // round ("snake" order) evens out the total width per wavefront.
+        const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
+        if (k >= nsl) continue;
+        const int s = s0 + k;
+        const int base = slice_off[s];
+        const int width = (slice_off[s + 1] - base) / kWave;
+        const double *vp = val + base + lane;
+        const uint16_t *cp = col + base + lane;
+        double acc = 0.0;
+        int j = 0;
+        for (; j + UNR <= width; j += UNR) {
+            double av[UNR];
+            int c[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                av[u] = vp[(size_t)(j + u) * kWave];
+                c[u] = cp[(size_t)(j + u) * kWave];
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; u++) acc = fma(av[u], vblk[c[u]], acc);
+        }
+        for (; j < width; j++) acc = fma(vp[(size_t)j * kWave], vblk[cp[(size_t)j * kWave]], acc);
+        const int row = slice_row[(size_t)s * kWave + lane];
+        if (row >= 0) part_b[row] = acc;
+        if constexpr (MODE == 0) {
+            const double d = segment_sum<64>(row >= 0 ? v[row] * acc : 0.0);
+            if (lane == 0) slice_dot[k] = d;
+        }
+    }
+    if constexpr (MODE == 0) {
+        __syncthreads();
+        double dot = 0.0;
+        for (int i = threadIdx.x; i < nsl; i += BLOCK) dot += slice_dot[i];
+        store_block_partial<BLOCK>(dot, dot_part);
+    }
+}
+
+// y = sum over blocks of the SELL SpMV's row partials (blocks in order).
+__global__ __launch_bounds__(kBlock) void combine_rows_kernel(
+    const double *__restrict__ rows_part, int nblk, double *__restrict__ y, int n) {
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        z[j] = z[j] + alpha * p[j];
-        const double rj = r[j] - alpha * q[j];
-        r[j] = rj;
-        acc[0] += rj * rj;
-    }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &sc->ticket[kTicketZR], tot) && threadIdx.x == 0) {
-        sc->rho = tot[0];
-        sc->beta = tot[0] / sc->rho0;
+        double s = rows_part[j];
+        for (int b = 1; b < nblk; b++) s += rows_part[(size_t)b * n + j];
+        y[j] = s;
     }
 }
 
-// p = r + beta p.
+// r = sum over blocks of the residual SpMV's row partials, with partials of
+// ||x - r||^2, x.z and z.z (stride gridDim.x).
+__global__ __launch_bounds__(kBlock) void resid_combine_kernel(
+    const double *__restrict__ rows_part, int nblk, const double *__restrict__ x,
+    const double *__restrict__ z, double *__restrict__ r, int n, double *__restrict__ res) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        double rj = rows_part[j];
+        for (int b = 1; b < nblk; b++) rj += rows_part[(size_t)b * n + j];
+        r[j] = rj;
+        const double xv = x[j], zv = z[j];
+        const double dlt = xv - rj;
+        a0 += dlt * dlt;
+        a1 += xv * zv;
+        a2 += zv * zv;
+    }
+    store_block_partial<kBlock>(a0, res);
+    __syncthreads();
+    store_block_partial<kBlock>(a1, res + gridDim.x);
+    __syncthreads();
+    store_block_partial<kBlock>(a2, res + 2 * gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise kernels
+// ---------------------------------------------------------------------------
+
+// z += alpha p; r -= alpha q with alpha = rho / (p.q); partials of new r.r.
+// rho comes from the partials of the previous r.r (or of x.x for the first
+// step), p.q from the SpMV's partials.
+// With nblk > 0, q is first assembled from the SELL SpMV's per-block row
+// partials (blocks added in order) and stored.
+__global__ __launch_bounds__(kBlock) void update_zr_kernel(
+    const double *__restrict__ p, double *__restrict__ q, double *__restrict__ z,
+    double *__restrict__ r, int n, const double *__restrict__ rho_part, int n_rho,
+    const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
+    const double *__restrict__ rows_part, int nblk, Scalars *sc) {
+    const double rho = reduce_partials<kBlock>(rho_part, n_rho);
+    const double d = reduce_partials<kBlock>(pq_part, n_pq);
+    const double alpha = rho / d;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->rho0 = rho;
+        sc->d = d;
+        sc->alpha = alpha;
+    }
+    double acc = 0.0;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        double qj;
+        if (nblk > 0) {
+            qj = rows_part[j];
+            for (int b = 1; b < nblk; b++) qj += rows_part[(size_t)b * n + j];
+            q[j] = qj;
+        } else {
+            qj = q[j];
+        }
+        z[j] = z[j] + alpha * p[j];
+        const double rj = r[j] - alpha * qj;
+        r[j] = rj;
+        acc += rj * rj;
+    }
+    store_block_partial<kBlock>(acc, rr_out);
+}
+
+// p = r + beta p with beta = (new r.r) / (old r.r), both from partials.
 __global__ __launch_bounds__(kBlock) void update_p_kernel(
     const double *__restrict__ r, double *__restrict__ p, int n,
-    const Scalars *__restrict__ sc) {
-    const double beta = sc->beta;
+    const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
+    Scalars *sc) {
+    const double rho = reduce_partials<kBlock>(rr_part, n_vec);
+    const double rho0 = reduce_partials<kBlock>(rho_part, n_vec);
+    const double beta = rho / rho0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->rho = rho;
+        sc->beta = beta;
+    }
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
         p[j] = r[j] + beta * p[j];
 }
 
 // Start of conj_grad (cg.cu:671-686): q = z = 0, r = p = x for j <= n and
-// rho = x.x over j < n.  If the previous outer iteration left a pending
-// normalisation, x = xscale * z (j < n) is applied first (cg.cu:479-481).
+// partials of x.x over j < n.  With scale_x, x = xscale * z (j < n) is
+// applied first: the normalisation of the previous outer step (:479-481).
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(
     double *__restrict__ x, double *__restrict__ z, double *__restrict__ p,
-    double *__restrict__ q, double *__restrict__ r, int n,
-    double *__restrict__ part, Scalars *sc) {
-    const bool scale = sc->pending_scale != 0;
-    const double xs = sc->xscale;
-    double acc[1] = {0.0};
+    double *__restrict__ q, double *__restrict__ r, int n, int scale_x,
+    const Scalars *__restrict__ sc, double *__restrict__ xx_out) {
+    const double xs = scale_x ? sc->xscale : 1.0;
+    double acc = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j <= n; j += gridDim.x * kBlock) {
         double xv = x[j];
-        if (scale && j < n) {
+        if (scale_x && j < n) {
             xv = xs * z[j];
             x[j] = xv;
         }
@@ -123,51 +293,28 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(
         z[j] = 0.0;
         r[j] = xv;
         p[j] = xv;
-        if (j < n) acc[0] += xv * xv;
+        if (j < n) acc += xv * xv;
     }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &sc->ticket[kTicketInit], tot) && threadIdx.x == 0) {
-        sc->rho = tot[0];
-        sc->pending_scale = 0;
-    }
+    store_block_partial<kBlock>(acc, xx_out);
 }
 
-// End of conj_grad and the outer step (cg.cu:782-800, :505-514):
-// r = A z, rnorm = ||x - r||, x.z, z.z, zeta = shift + 1/(x.z),
-// xscale = 1/sqrt(z.z).  With record != 0 the (zeta, rnorm) pair is stored
-// at history index sc->iter and a normalisation of x is left pending.
-template <int SEG, int UNR>
-__global__ __launch_bounds__(kBlock) void resid_kernel(
-    const int *__restrict__ rowstr, const int *__restrict__ colidx,
-    const double *__restrict__ a, const double *__restrict__ x,
-    const double *__restrict__ z, double *__restrict__ r, int n,
-    double *__restrict__ part, Scalars *sc, double *__restrict__ zeta_hist,
-    double *__restrict__ rnorm_hist, int hist_cap, int record) {
-    constexpr int kGroups = kBlock / SEG;
-    const int sl = threadIdx.x % SEG;
-    const int g = blockIdx.x * kGroups + threadIdx.x / SEG;
-    const int ng = gridDim.x * kGroups;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int row = g; row < n; row += ng) {
-        const int s = rowstr[row], e = rowstr[row + 1];
-        const double sum = row_dot<SEG, UNR>(colidx, a, z, s, e, sl);
-        if (sl == 0) {
-            r[row] = sum;
-            const double xv = x[row], zv = z[row];
-            const double dlt = xv - sum;
-            acc[0] += dlt * dlt;
-            acc[1] += xv * zv;
-            acc[2] += zv * zv;
-        }
-    }
-    double tot[3];
-    if (grid_reduce<3>(acc, part, &sc->ticket[kTicketResid], tot) && threadIdx.x == 0) {
-        const double rnorm = sqrt(tot[0]);
+// One workgroup: reduce the residual SpMV's partials and form
+// rnorm = ||x - A z|| (cg.cu:795-800), norm_temp1 = x.z, norm_temp2 =
+// 1/sqrt(z.z) and zeta = shift + 1/(x.z) (:505-514).
+__global__ __launch_bounds__(kBlock) void resid_finish_kernel(
+    const double *__restrict__ res, int count, Scalars *sc,
+    double *__restrict__ zeta_hist, double *__restrict__ rnorm_hist, int hist_cap,
+    int record) {
+    const double s0 = reduce_partials<kBlock>(res, count);
+    const double s1 = reduce_partials<kBlock>(res + count, count);
+    const double s2 = reduce_partials<kBlock>(res + 2 * count, count);
+    if (threadIdx.x == 0) {
+        const double rnorm = sqrt(s0);
         sc->rnorm = rnorm;
-        sc->xz = tot[1];
-        sc->zz = tot[2];
-        sc->xscale = 1.0 / sqrt(tot[2]);
-        const double zeta = sc->shift + 1.0 / tot[1];
+        sc->xz = s1;
+        sc->zz = s2;
+        sc->xscale = 1.0 / sqrt(s2);
+        const double zeta = sc->shift + 1.0 / s1;
         sc->zeta = zeta;
         if (record) {
             const int it = sc->iter;
@@ -176,53 +323,51 @@ __global__ __launch_bounds__(kBlock) void resid_kernel(
                 rnorm_hist[it] = rnorm;
             }
             sc->iter = it + 1;
-            sc->pending_scale = 1;
         }
     }
 }
 
-// x.z and z.z (cg.cu:1182-1245, gpu_kernel_ten).
+// Partials of x.z and z.z (cg.cu:1182-1245, gpu_kernel_ten).
 __global__ __launch_bounds__(kBlock) void norms_kernel(
     const double *__restrict__ x, const double *__restrict__ z, int n,
-    double *__restrict__ part, Scalars *sc) {
-    double acc[2] = {0.0, 0.0};
+    double *__restrict__ part) {
+    double a0 = 0.0, a1 = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         const double zv = z[j];
-        acc[0] += x[j] * zv;
-        acc[1] += zv * zv;
+        a0 += x[j] * zv;
+        a1 += zv * zv;
     }
-    double tot[2];
-    if (grid_reduce<2>(acc, part, &sc->ticket[kTicketNorms], tot) && threadIdx.x == 0) {
-        sc->xz = tot[0];
-        sc->zz = tot[1];
+    store_block_partial<kBlock>(a0, part);
+    __syncthreads();
+    store_block_partial<kBlock>(a1, part + gridDim.x);
+}
+
+__global__ __launch_bounds__(kBlock) void norms_finish_kernel(
+    const double *__restrict__ part, int count, Scalars *sc) {
+    const double s0 = reduce_partials<kBlock>(part, count);
+    const double s1 = reduce_partials<kBlock>(part + count, count);
+    if (threadIdx.x == 0) {
+        sc->xz = s0;
+        sc->zz = s1;
     }
 }
 
-// x = s * z for j < n (cg.cu:1261-1265, gpu_kernel_eleven).  With
-// from_pending != 0 the scale is sc->xscale and only applied if pending.
+// x = s * z for j < n (cg.cu:1261-1265, gpu_kernel_eleven); with
+// from_scalars the scale is the last residual finish's xscale.
 __global__ __launch_bounds__(kBlock) void scale_x_kernel(
     double *__restrict__ x, const double *__restrict__ z, int n, double s,
-    int from_pending, Scalars *sc) {
-    if (from_pending) {
-        if (sc->pending_scale == 0) return;
-        s = sc->xscale;
-    }
+    int from_scalars, const Scalars *__restrict__ sc) {
+    if (from_scalars) s = sc->xscale;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
         x[j] = s * z[j];
 }
 
-__global__ void clear_pending_kernel(Scalars *sc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) sc->pending_scale = 0;
-}
-
-// Resets the outer-loop bookkeeping: history index, shift, pending flag and
-// all tickets.  A kernel rather than a memset so it may sit in a graph.
+// Resets the outer-loop bookkeeping.  A kernel rather than a memset so it may
+// sit in a captured graph.
 __global__ void reset_scalars_kernel(Scalars *sc, double shift) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         sc->iter = 0;
         sc->shift = shift;
-        sc->pending_scale = 0;
-        for (int i = 0; i < 8; i++) sc->ticket[i] = 0u;
     }
 }
 
@@ -232,180 +377,71 @@ __global__ void fill_kernel(double *__restrict__ v, int count, double value) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged SpMV over the column-blocked copy (tiled_layout.h).
-//
-// One 1024-thread workgroup owns a tile of T rows.  For each column block it
-// stages the W entries of the input vector the block covers into LDS with
-// coalesced 16-byte loads, then groups of SEG lanes sweep the tile's row
-// segments in that block, streaming (value, 16-bit column) pairs from HBM and
-// gathering the vector from LDS.  Each row's running sum lives in LDS and is
-// owned by one lane group, so the per-row summation order (block by block,
-// then lane-strided with a fixed DPP tree) is fixed.  The random 8-byte
-// gathers that bound the CSR kernel (one 128-byte L2 line per gather) become
-// LDS reads; the vector crosses from L2 once per tile and block.
-// ---------------------------------------------------------------------------
-
-template <int SEG, int UNR>
-__device__ __forceinline__ double seg_dot_lds(const double *__restrict__ val,
-                                              const uint16_t *__restrict__ col,
-                                              const double *vblk, int s, int e, int sl) {
-    double sum = 0.0;
-    for (int k0 = s + sl; k0 < e; k0 += SEG * UNR) {
-        double av[UNR];
-        int c[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; u++) {
-            const int k = k0 + u * SEG;
-            const bool ok = k < e;
-            av[u] = ok ? val[k] : 0.0;
-            c[u] = ok ? (int)col[k] : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; u++)
-            if (c[u] >= 0) sum = fma(av[u], vblk[c[u]], sum);
-    }
-    return segment_sum<SEG>(sum);
-}
-
-// MODE 0: y = A v with v = p, y = q; epilogue d = p.q, alpha = rho/d.
-// MODE 1: y = A v with v = z, y = r; epilogue ||x - r||, x.z, z.z, zeta.
-template <int SEG, int UNR, int MODE>
-__global__ __launch_bounds__(kTiledBlock) void spmv_tiled_kernel(
-    const int *__restrict__ seg, const double *__restrict__ val,
-    const uint16_t *__restrict__ col, const double *__restrict__ v,
-    double *__restrict__ y, const double *__restrict__ x, int n, int T, int W, int nblk,
-    int ntile, double *__restrict__ part, Scalars *sc, double *__restrict__ zeta_hist,
-    double *__restrict__ rnorm_hist, int hist_cap, int record) {
-    extern __shared__ double2 lds_v2[];  // 16-byte aligned dynamic LDS
-    double *lds = reinterpret_cast<double *>(lds_v2);
-    double *vblk = lds;       // W entries of the input vector
-    double *rowsum = lds + W; // T running row sums
-    constexpr int G = kTiledBlock / SEG;
-    const int g = threadIdx.x / SEG, sl = threadIdx.x % SEG;
-    double acc[MODE == 0 ? 1 : 3] = {};
-    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
-        const int row0 = t * T;
-        const int nrows = min(T, n - row0);
-        for (int i = threadIdx.x; i < T; i += kTiledBlock) rowsum[i] = 0.0;
-        for (int b = 0; b < nblk; b++) {
-            const int c0 = b * W;
-            const int wb = min(W, n - c0);
-            __syncthreads();  // previous block's gathers are done with vblk
-            const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
-            double2 *l2 = reinterpret_cast<double2 *>(vblk);
-#pragma unroll 4
-            for (int i = threadIdx.x; i < wb / 2; i += kTiledBlock) l2[i] = v2[i];
-            if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
-            __syncthreads();
-            const int *sp = seg + (size_t)(t * nblk + b) * (T + 1);
-            for (int r = g; r < nrows; r += G) {
-                const double s = seg_dot_lds<SEG, UNR>(val, col, vblk, sp[r], sp[r + 1], sl);
-                if (sl == 0) rowsum[r] += s;
-            }
-        }
-        __syncthreads();
-        for (int r = threadIdx.x; r < nrows; r += kTiledBlock) {
-            const int row = row0 + r;
-            const double yv = rowsum[r];
-            y[row] = yv;
-            if constexpr (MODE == 0) {
-                acc[0] += v[row] * yv;
-            } else {
-                const double xv = x[row], zv = v[row];
-                const double dlt = xv - yv;
-                acc[0] += dlt * dlt;
-                acc[1] += xv * zv;
-                acc[2] += zv * zv;
-            }
-        }
-        __syncthreads();  // rowsum is reset for the next tile
-    }
-    double tot[MODE == 0 ? 1 : 3];
-    if constexpr (MODE == 0) {
-        if (grid_reduce<1, kTiledBlock>(acc, part, &sc->ticket[kTicketSpmv], tot) &&
-            threadIdx.x == 0) {
-            sc->d = tot[0];
-            sc->alpha = sc->rho / tot[0];
-            sc->rho0 = sc->rho;
-        }
-    } else {
-        if (grid_reduce<3, kTiledBlock>(acc, part, &sc->ticket[kTicketResid], tot) &&
-            threadIdx.x == 0) {
-            const double rnorm = sqrt(tot[0]);
-            sc->rnorm = rnorm;
-            sc->xz = tot[1];
-            sc->zz = tot[2];
-            sc->xscale = 1.0 / sqrt(tot[2]);
-            const double zeta = sc->shift + 1.0 / tot[1];
-            sc->zeta = zeta;
-            if (record) {
-                const int it = sc->iter;
-                if (it < hist_cap) {
-                    zeta_hist[it] = zeta;
-                    rnorm_hist[it] = rnorm;
-                }
-                sc->iter = it + 1;
-                sc->pending_scale = 1;
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
 
 namespace {
 
-template <int SEG, int UNR>
-hipError_t launch_spmv_pq_t(const Fp64Problem &pb, hipStream_t st) {
-    spmv_pq_kernel<SEG, UNR><<<pb.spmv_blocks, kBlock, 0, st>>>(
-        pb.rowstr, pb.colidx, pb.a, pb.p, pb.q, pb.n, pb.part, pb.sc);
-    return hipGetLastError();
-}
-
-template <int SEG, int UNR>
-hipError_t launch_resid_t(const Fp64Problem &pb, int record, hipStream_t st) {
-    resid_kernel<SEG, UNR><<<pb.spmv_blocks, kBlock, 0, st>>>(
-        pb.rowstr, pb.colidx, pb.a, pb.x, pb.z, pb.r, pb.n, pb.part, pb.sc,
-        pb.zeta_hist, pb.rnorm_hist, pb.hist_cap, record);
-    return hipGetLastError();
-}
-
-template <int SEG, int MODE>
-hipError_t launch_tiled_t(const Fp64Problem &pb, int record, hipStream_t st) {
-    const TiledDev &td = pb.tiled;
+template <int SEG, int UNR, int MODE>
+hipError_t launch_csr_t(const Fp64Problem &pb, hipStream_t st) {
     const double *v = MODE == 0 ? pb.p : pb.z;
     double *y = MODE == 0 ? pb.q : pb.r;
-    spmv_tiled_kernel<SEG, 2, MODE><<<td.grid, kTiledBlock, td.lds_bytes, st>>>(
-        td.seg, td.val, td.col, v, y, pb.x, pb.n, td.T, td.W, td.nblk, td.ntile, pb.part,
-        pb.sc, pb.zeta_hist, pb.rnorm_hist, pb.hist_cap, record);
+    double *part = MODE == 0 ? pb.part.pq : pb.part.res;
+    spmv_csr_kernel<SEG, UNR, MODE><<<pb.spmv_blocks, kBlock, 0, st>>>(
+        pb.rowstr, pb.colidx, pb.a, v, y, pb.x, pb.n, part);
+    return hipGetLastError();
+}
+
+template <int BLOCK, int MODE>
+hipError_t launch_sell_t(const Fp64Problem &pb, hipStream_t st) {
+    const SellDev &d = pb.sell;
+    const double *v = MODE == 0 ? pb.p : pb.z;
+    spmv_sell_kernel<BLOCK, 8, MODE><<<d.grid, BLOCK, d.lds_bytes, st>>>(
+        d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v, d.rows_part, pb.part.pq, pb.n,
+        d.W, d.nrange);
     return hipGetLastError();
 }
 
 template <int MODE>
-hipError_t launch_tiled(const Fp64Problem &pb, int record, hipStream_t st) {
-    switch (pb.tiled.seg_lanes) {
-    case 16: return launch_tiled_t<16, MODE>(pb, record, st);
-    case 64: return launch_tiled_t<64, MODE>(pb, record, st);
-    default: return launch_tiled_t<32, MODE>(pb, record, st);
+hipError_t launch_spmv(const Fp64Problem &pb, hipStream_t st) {
+    switch (pb.spmv_variant) {
+    case kSpmvTiled:
+        return pb.sell.block == 512 ? launch_sell_t<512, MODE>(pb, st)
+                                    : launch_sell_t<1024, MODE>(pb, st);
+    case kSpmvSeg16: return launch_csr_t<16, 4, MODE>(pb, st);
+    case kSpmvSeg32: return launch_csr_t<32, 4, MODE>(pb, st);
+    case kSpmvSeg64U2: return launch_csr_t<64, 2, MODE>(pb, st);
+    default: return launch_csr_t<64, 4, MODE>(pb, st);
     }
 }
 
-template <int SEG, int MODE>
-hipError_t set_tiled_lds_t(int bytes) {
+template <int BLOCK, int MODE>
+hipError_t set_sell_lds_t(int bytes) {
     return hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&spmv_tiled_kernel<SEG, 2, MODE>),
+        reinterpret_cast<const void *>(&spmv_sell_kernel<BLOCK, 8, MODE>),
         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// The rho partials feeding CG step `step`: x.x from init for step 0, else
+// the r.r written by the previous step's update_zr.
+const double *rho_source(const Fp64Problem &pb, int step) {
+    return step == 0 ? pb.part.xx : pb.part.rr[(step - 1) & 1];
 }
 
 }  // namespace
 
-hipError_t prepare_tiled_kernels(int lds_bytes) {
+int spmv_grid(const Fp64Problem &pb) {
+    return pb.spmv_variant == kSpmvTiled ? pb.sell.grid : pb.spmv_blocks;
+}
+
+int resid_partials(const Fp64Problem &pb) {
+    return pb.spmv_variant == kSpmvTiled ? pb.vec_blocks : pb.spmv_blocks;
+}
+
+hipError_t prepare_sell_kernels(int lds_bytes) {
     hipError_t e = hipSuccess;
-    hipError_t r[6] = {set_tiled_lds_t<16, 0>(lds_bytes), set_tiled_lds_t<32, 0>(lds_bytes),
-                       set_tiled_lds_t<64, 0>(lds_bytes), set_tiled_lds_t<16, 1>(lds_bytes),
-                       set_tiled_lds_t<32, 1>(lds_bytes), set_tiled_lds_t<64, 1>(lds_bytes)};
+    hipError_t r[4] = {set_sell_lds_t<512, 0>(lds_bytes), set_sell_lds_t<1024, 0>(lds_bytes),
+                       set_sell_lds_t<512, 1>(lds_bytes), set_sell_lds_t<1024, 1>(lds_bytes)};
     for (hipError_t x : r)
         if (x != hipSuccess) e = x;
     return e;
@@ -420,56 +456,65 @@ int spmv_rows_per_block(int variant) {
 }
 
 hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st) {
-    switch (pb.spmv_variant) {
-    case kSpmvTiled: return launch_tiled<0>(pb, 0, st);
-    case kSpmvSeg16: return launch_spmv_pq_t<16, 4>(pb, st);
-    case kSpmvSeg32: return launch_spmv_pq_t<32, 4>(pb, st);
-    case kSpmvSeg64U2: return launch_spmv_pq_t<64, 2>(pb, st);
-    default: return launch_spmv_pq_t<64, 4>(pb, st);
-    }
+    return launch_spmv<0>(pb, st);
+}
+
+hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st) {
+    hipError_t e = launch_spmv<0>(pb, st);
+    if (e != hipSuccess || pb.spmv_variant != kSpmvTiled) return e;
+    combine_rows_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.sell.rows_part, pb.sell.nblk,
+                                                          pb.q, pb.n);
+    return hipGetLastError();
 }
 
 hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
-    switch (pb.spmv_variant) {
-    case kSpmvTiled: return launch_tiled<1>(pb, record, st);
-    case kSpmvSeg16: return launch_resid_t<16, 4>(pb, record, st);
-    case kSpmvSeg32: return launch_resid_t<32, 4>(pb, record, st);
-    case kSpmvSeg64U2: return launch_resid_t<64, 2>(pb, record, st);
-    default: return launch_resid_t<64, 4>(pb, record, st);
+    hipError_t e = launch_spmv<1>(pb, st);
+    if (e != hipSuccess) return e;
+    if (pb.spmv_variant == kSpmvTiled) {
+        resid_combine_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+            pb.sell.rows_part, pb.sell.nblk, pb.x, pb.z, pb.r, pb.n, pb.part.res);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-}
-
-hipError_t launch_update_zr(const Fp64Problem &pb, hipStream_t st) {
-    update_zr_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.p, pb.q, pb.z, pb.r, pb.n,
-                                                       pb.part, pb.sc);
+    resid_finish_kernel<<<1, kBlock, 0, st>>>(pb.part.res, resid_partials(pb), pb.sc,
+                                              pb.zeta_hist, pb.rnorm_hist, pb.hist_cap,
+                                              record);
     return hipGetLastError();
 }
 
-hipError_t launch_update_p(const Fp64Problem &pb, hipStream_t st) {
-    update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.r, pb.p, pb.n, pb.sc);
+hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st) {
+    cg_init_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.p, pb.q, pb.r, pb.n,
+                                                     scale_x, pb.sc, pb.part.xx);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_init(const Fp64Problem &pb, hipStream_t st) {
-    cg_init_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.p, pb.q, pb.r,
-                                                     pb.n, pb.part, pb.sc);
+hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st) {
+    update_zr_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+        pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, step), pb.vec_blocks, pb.part.pq,
+        spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part,
+        pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st) {
+    update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.r, pb.p, pb.n, pb.part.rr[step & 1],
+                                                      rho_source(pb, step), pb.vec_blocks,
+                                                      pb.sc);
     return hipGetLastError();
 }
 
 hipError_t launch_norms(const Fp64Problem &pb, hipStream_t st) {
-    norms_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.n, pb.part, pb.sc);
+    norms_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.n, pb.part.nrm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    norms_finish_kernel<<<1, kBlock, 0, st>>>(pb.part.nrm, pb.vec_blocks, pb.sc);
     return hipGetLastError();
 }
 
-hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_pending,
+hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_scalars,
                           hipStream_t st) {
-    scale_x_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.n, s, from_pending,
+    scale_x_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.n, s, from_scalars,
                                                      pb.sc);
-    return hipGetLastError();
-}
-
-hipError_t launch_clear_pending(const Fp64Problem &pb, hipStream_t st) {
-    clear_pending_kernel<<<1, 64, 0, st>>>(pb.sc);
     return hipGetLastError();
 }
 
@@ -483,24 +528,21 @@ hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream
     return hipGetLastError();
 }
 
-// One conj_grad (cg.cu:653-801) as a launch sequence on `st`.
-hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int record,
+hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, int record,
                              hipStream_t st, const SpmvProbe *probe) {
-    hipError_t e = launch_cg_init(pb, st);
+    hipError_t e = launch_cg_init(pb, scale_x, st);
     for (int it = 0; it < cgitmax && e == hipSuccess; it++) {
         const bool timed = probe && probe->start && it < probe->count;
         if (timed) e = hipEventRecord(probe->start[it], st);
         if (e == hipSuccess) e = launch_spmv_pq(pb, st);
         if (timed && e == hipSuccess) e = hipEventRecord(probe->stop[it], st);
-        if (e == hipSuccess) e = launch_update_zr(pb, st);
-        if (e == hipSuccess) e = launch_update_p(pb, st);
+        if (e == hipSuccess) e = launch_update_zr(pb, it, st);
+        if (e == hipSuccess) e = launch_update_p(pb, it, st);
     }
     if (e == hipSuccess) e = launch_resid(pb, record, st);
     return e;
 }
 
-}  // namespace npbcg_dev
-
-namespace npbcg_dev {
 size_t scalars_bytes() { return sizeof(Scalars); }
+
 }  // namespace npbcg_dev

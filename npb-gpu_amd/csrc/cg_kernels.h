@@ -2,34 +2,46 @@
 #ifndef NPBCG_CG_KERNELS_H
 #define NPBCG_CG_KERNELS_H
 
+#include <cstdint>
+
 #include <hip/hip_runtime.h>
 
 namespace npbcg_dev {
 
 struct Scalars;
 
-// SpMV kernel shapes: lanes per CSR row and loads in flight per lane.
+// SpMV kernel shapes.
 enum SpmvVariant {
     kSpmvAuto = 0,
-    kSpmvSeg64 = 1,    // one wavefront per row, 4 loads in flight per lane
-    kSpmvSeg32 = 2,    // half-wavefront per row
-    kSpmvSeg16 = 3,    // quarter-wavefront per row
-    kSpmvSeg64U2 = 4,  // one wavefront per row, 2 loads in flight per lane
+    kSpmvSeg64 = 1,    // CSR, one wavefront per row, 4 loads in flight per lane
+    kSpmvSeg32 = 2,    // CSR, half-wavefront per row
+    kSpmvSeg16 = 3,    // CSR, quarter-wavefront per row
+    kSpmvSeg64U2 = 4,  // CSR, one wavefront per row, 2 loads in flight per lane
     kSpmvTiled = 5     // column-blocked copy, vector blocks staged in LDS
 };
 
-constexpr int kTiledBlock = 1024;  // threads per workgroup of the tiled SpMV
-
-// Device copy of the column-blocked matrix (tiled_layout.h) and its launch
-// shape.
-struct TiledDev {
-    const int *seg = nullptr;
+// Device copy of the column-blocked SELL matrix (tiled_layout.h) and its
+// launch shape.
+struct SellDev {
+    const int *wg_slice = nullptr;
+    const int *slice_off = nullptr;
+    const int *slice_row = nullptr;
     const double *val = nullptr;
     const uint16_t *col = nullptr;
-    int T = 0, W = 0, nblk = 0, ntile = 0;
-    int grid = 0;       // workgroups launched (grid-stride over tiles)
-    int seg_lanes = 32; // lanes per row segment: 16, 32 or 64
-    int lds_bytes = 0;  // dynamic LDS: (W + T) doubles
+    double *rows_part = nullptr;  // nblk * n row sums, one set per column block
+    int W = 0, nblk = 0, R = 0, nrange = 0;
+    int grid = 0;       // nblk * nrange workgroups
+    int block = 1024;   // threads per workgroup: 1024 or 512
+    int lds_bytes = 0;  // dynamic LDS: W vector entries + one double per slice
+};
+
+// Per-workgroup partial sums handed from one kernel to the next.
+struct Partials {
+    double *xx;      // x.x of cg_init            [vec_blocks]
+    double *rr[2];   // r.r of update_zr, alternating by CG step [vec_blocks]
+    double *pq;      // p.q of the SpMV           [spmv_grid]
+    double *res;     // ||x-Az||^2, x.z, z.z of the residual [3 * max(spmv_grid, vec_blocks)]
+    double *nrm;     // x.z, z.z of norms_kernel  [2 * vec_blocks]
 };
 
 // Device pointers and launch shapes of one FP64 problem.
@@ -40,15 +52,18 @@ struct Fp64Problem {
     const int *colidx;
     const double *a;
     double *x, *z, *p, *q, *r;  // n+2 entries each
-    double *part;               // reduction partials, 3 * max(blocks)
+    Partials part;
     Scalars *sc;
     double *zeta_hist, *rnorm_hist;
     int hist_cap;
     int vec_blocks;             // workgroups of the element-wise kernels
-    int spmv_blocks;            // workgroups of the SpMV kernels
+    int spmv_blocks;            // workgroups of the CSR SpMV kernels
     int spmv_variant;
-    TiledDev tiled;
+    SellDev sell;
 };
+
+// Workgroups launched by the SpMV of this problem (= partials it writes).
+int spmv_grid(const Fp64Problem &pb);
 
 // Optional HIP events bracketing the first `count` SpMV launches of a
 // conj_grad call (used to time the SpMV kernel inside a timed run).
@@ -60,25 +75,30 @@ struct SpmvProbe {
 
 int spmv_rows_per_block(int variant);
 
-// Raises the dynamic-LDS limit of the tiled SpMV kernels to lds_bytes.
-hipError_t prepare_tiled_kernels(int lds_bytes);
+// Raises the dynamic-LDS limit of the SELL SpMV kernels to lds_bytes.
+hipError_t prepare_sell_kernels(int lds_bytes);
 
+// q = A p (partials p.q).  With the SELL layout q itself is assembled later,
+// by update_zr, from the per-block row sums.
 hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st);
+// A complete q = A p for every layout (SELL: SpMV then block combine).
+hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st);
+// r = A z (partials ||x - r||^2, x.z, z.z), then a one-workgroup finish that
+// forms rnorm, zeta, xscale and, with record != 0, appends to the history.
 hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st);
-hipError_t launch_update_zr(const Fp64Problem &pb, hipStream_t st);
-hipError_t launch_update_p(const Fp64Problem &pb, hipStream_t st);
-hipError_t launch_cg_init(const Fp64Problem &pb, hipStream_t st);
+hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st);
+hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st);
+hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st);
 hipError_t launch_norms(const Fp64Problem &pb, hipStream_t st);
-hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_pending,
+// x = s * z, or x = xscale * z (the last residual finish) if from_scalars.
+hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_scalars,
                           hipStream_t st);
-hipError_t launch_clear_pending(const Fp64Problem &pb, hipStream_t st);
 hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st);
 hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream_t st);
 
-// One conj_grad: init, cgitmax x (spmv_pq, update_zr, update_p), resid.
-// record != 0 stores (zeta, rnorm) in the history and leaves x's
-// normalisation pending for the next init.
-hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int record,
+// One conj_grad: init (optionally applying x = xscale z first), cgitmax x
+// (spmv, update_zr, update_p), residual SpMV and finish.
+hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, int record,
                              hipStream_t st, const SpmvProbe *probe);
 
 size_t scalars_bytes();

@@ -1,4 +1,4 @@
-// Host construction of the column-blocked copy used by the LDS-staged SpMV.
+// Host construction of the column-blocked SELL copy (tiled_layout.h).
 #include "tiled_layout.h"
 
 #include <algorithm>
@@ -9,93 +9,122 @@ namespace npbcg_detail {
 namespace {
 
 template <class F>
-void for_tiles(int ntile, F &&body) {
+void parallel_items(int count, F &&body) {
     unsigned hc = std::thread::hardware_concurrency();
     int nth = (int)std::min<unsigned>(hc ? hc : 1, 16u);
-    nth = std::min(nth, ntile);
+    nth = std::min(nth, count);
     if (nth <= 1) {
-        for (int t = 0; t < ntile; t++) body(t);
+        for (int i = 0; i < count; i++) body(i);
         return;
     }
     std::vector<std::thread> pool;
     for (int w = 0; w < nth; w++)
         pool.emplace_back([&, w] {
-            for (int t = w; t < ntile; t += nth) body(t);
+            for (int i = w; i < count; i += nth) body(i);
         });
     for (auto &th : pool) th.join();
 }
 
+// Rows of range rr ordered by their segment length in block b (longest
+// first, ties by row), with the segment bounds.
+struct RowSeg {
+    int row, lo, hi;
+};
+
+void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, int b,
+                    int rr, std::vector<RowSeg> *out) {
+    out->clear();
+    const int r0 = rr * R, r1 = std::min(n, r0 + R);
+    const int c0 = b * W, c1 = c0 + W;
+    for (int j = r0; j < r1; j++) {
+        const int *first = colidx + rowstr[j], *last = colidx + rowstr[j + 1];
+        const int lo = (int)(std::lower_bound(first, last, c0) - colidx);
+        const int hi = (int)(std::lower_bound(first, last, c1) - colidx);
+        out->push_back({j, lo, hi});
+    }
+    std::stable_sort(out->begin(), out->end(), [](const RowSeg &x, const RowSeg &y) {
+        return (x.hi - x.lo) > (y.hi - y.lo);
+    });
+}
+
 template <class V>
-bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, int T, int W,
-                TiledMatrix<V> *out) {
-    if (n < 1 || T < 1 || W < 1 || W > 65536) return false;
-    TiledMatrix<V> &m = *out;
+bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, int W, int R,
+                SellBlocks<V> *out) {
+    if (n < 1 || W < 1 || W > 65536 || R < 1) return false;
+    SellBlocks<V> &m = *out;
     m.n = n;
-    m.T = T;
     m.W = W;
+    m.R = R;
     m.nblk = (n + W - 1) / W;
-    m.ntile = (n + T - 1) / T;
+    m.nrange = (n + R - 1) / R;
     m.nnz = rowstr[n];
-    const int nblk = m.nblk, ntile = m.ntile;
-    const size_t segs = (size_t)ntile * nblk * (T + 1);
-    if (segs > (size_t)INT32_MAX) return false;
+    const int nwg = m.nblk * m.nrange;
 
-    std::vector<long long> cnt((size_t)ntile * nblk, 0);
-    for_tiles(ntile, [&](int t) {
-        for (int i = 0; i < T; i++) {
-            const int row = t * T + i;
-            if (row >= n) break;
-            for (int k = rowstr[row]; k < rowstr[row + 1]; k++)
-                cnt[(size_t)t * nblk + colidx[k] / W]++;
-        }
+    // pass 1: slices and stored entries per workgroup
+    std::vector<long long> wg_entries(nwg, 0);
+    std::vector<int> wg_slices(nwg, 0);
+    parallel_items(nwg, [&](int wg) {
+        std::vector<RowSeg> segs;
+        const int b = wg / m.nrange, rr = wg % m.nrange;
+        block_segments(n, rowstr, colidx, W, R, b, rr, &segs);
+        long long e = 0;
+        int s = 0;
+        for (size_t i = 0; i < segs.size(); i += kSliceRows, s++)
+            e += (long long)kSliceRows * (segs[i].hi - segs[i].lo);
+        wg_entries[wg] = e;
+        wg_slices[wg] = s;
     });
-    std::vector<long long> base(cnt.size() + 1, 0);
-    for (size_t i = 0; i < cnt.size(); i++) base[i + 1] = base[i] + cnt[i];
-    if (base.back() != m.nnz || m.nnz > (long long)INT32_MAX) return false;
+    m.wg_slice.assign(nwg + 1, 0);
+    std::vector<long long> wg_base(nwg + 1, 0);
+    for (int wg = 0; wg < nwg; wg++) {
+        m.wg_slice[wg + 1] = m.wg_slice[wg] + wg_slices[wg];
+        wg_base[wg + 1] = wg_base[wg] + wg_entries[wg];
+    }
+    m.stored = wg_base[nwg];
+    if (m.stored > (long long)INT32_MAX) return false;
+    const int nslices = m.wg_slice[nwg];
+    m.slice_off.assign(nslices + 1, 0);
+    m.slice_row.assign((size_t)nslices * kSliceRows, -1);
+    m.val.assign((size_t)m.stored, V(0));
+    m.col.assign((size_t)m.stored, 0);
 
-    m.seg.assign(segs, 0);
-    m.val.resize((size_t)m.nnz);
-    m.col.resize((size_t)m.nnz);
-    for_tiles(ntile, [&](int t) {
-        std::vector<int> cursor(T);
-        for (int i = 0; i < T; i++) {
-            const int row = t * T + i;
-            cursor[i] = row < n ? rowstr[row] : 0;
-        }
-        for (int b = 0; b < nblk; b++) {
-            long long off = base[(size_t)t * nblk + b];
-            int *sg = &m.seg[((size_t)t * nblk + b) * (T + 1)];
-            const int lim = (b + 1) * W;  // b*W <= col < lim
-            for (int i = 0; i < T; i++) {
-                sg[i] = (int)off;
-                const int row = t * T + i;
-                if (row >= n) continue;
-                int k = cursor[i];
-                const int e = rowstr[row + 1];
-                while (k < e && colidx[k] < lim) {
-                    m.val[off] = (V)a[k];
-                    m.col[off] = (uint16_t)(colidx[k] - b * W);
-                    off++;
-                    k++;
+    // pass 2: fill
+    parallel_items(nwg, [&](int wg) {
+        std::vector<RowSeg> segs;
+        const int b = wg / m.nrange, rr = wg % m.nrange;
+        block_segments(n, rowstr, colidx, W, R, b, rr, &segs);
+        long long off = wg_base[wg];
+        int s = m.wg_slice[wg];
+        for (size_t i0 = 0; i0 < segs.size(); i0 += kSliceRows, s++) {
+            const int width = segs[i0].hi - segs[i0].lo;
+            m.slice_off[s] = (int)off;
+            for (int l = 0; l < kSliceRows && i0 + l < segs.size(); l++) {
+                const RowSeg &rs = segs[i0 + l];
+                m.slice_row[(size_t)s * kSliceRows + l] = rs.row;
+                for (int j = 0; j < rs.hi - rs.lo; j++) {
+                    const size_t at = (size_t)off + (size_t)j * kSliceRows + l;
+                    m.val[at] = (V)a[rs.lo + j];
+                    m.col[at] = (uint16_t)(colidx[rs.lo + j] - b * W);
                 }
-                cursor[i] = k;
             }
-            sg[T] = (int)off;
+            off += (long long)kSliceRows * width;
         }
+        if (wg == nwg - 1) m.slice_off[nslices] = (int)off;
     });
+    if (nslices == 0) m.slice_off[0] = 0;
     return true;
 }
 
 }  // namespace
 
-bool build_tiled(int n, const int *rowstr, const int *colidx, const double *a, int T, int W,
-                 TiledMatrix<double> *out) {
-    return build_impl(n, rowstr, colidx, a, T, W, out);
+bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
+                       int R, SellBlocks<double> *out) {
+    return build_impl(n, rowstr, colidx, a, W, R, out);
 }
 
-bool build_tiled(int n, const int *rowstr, const int *colidx, const double *a, int T, int W,
-                 TiledMatrix<float> *out) {
-    return build_impl(n, rowstr, colidx, a, T, W, out);
+bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
+                       int R, SellBlocks<float> *out) {
+    return build_impl(n, rowstr, colidx, a, W, R, out);
 }
 
 }  // namespace npbcg_detail

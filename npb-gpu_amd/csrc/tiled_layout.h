@@ -1,12 +1,18 @@
-// Column-blocked ("tiled") copy of a CSR matrix for the LDS-staged SpMV.
+// Column-blocked SELL copy of a CSR matrix for the LDS-staged SpMV.
 //
-// Rows are cut into tiles of T rows and columns into blocks of W columns.
-// Storage order is tile -> block -> row -> column, so the part of a tile that
-// falls into one column block is contiguous.  Column indices are stored as
-// 16-bit offsets inside their block (W <= 65536).  seg[(t*nblk + b)*(T+1) + i]
-// is the offset of row t*T+i's segment in block b; entry T ends the last row.
-// Rows past n in the last tile have empty segments.  Within a row the
-// entries keep their CSR (ascending column) order.
+// Columns are cut into nblk blocks of W columns and rows into nrange ranges
+// of R rows.  Workgroup wg = b * nrange + rr owns the part of row range rr that
+// falls into column block b.  Inside a workgroup the rows are ordered by their
+// segment length in block b (longest first, ties by row) and cut into slices
+// of 64 rows; slice s stores its entries column-major: entry j of the slice's
+// lane l sits at slice_off[s] + 64*j + l, and rows shorter than the slice
+// width are padded with (value 0, column 0).  So one wavefront streams a slice
+// with fully coalesced loads, one row per lane, and no cross-lane reduction.
+// Column indices are 16-bit offsets inside the block (W <= 65536).
+//   wg_slice[wg] .. wg_slice[wg+1]   slices of workgroup wg
+//   slice_off[s] .. slice_off[s+1]   entries of slice s (multiple of 64)
+//   slice_row[64*s + l]              row (global) of lane l, -1 for padding
+// Within a row the entries keep their CSR (ascending column) order.
 #ifndef NPBCG_TILED_LAYOUT_H
 #define NPBCG_TILED_LAYOUT_H
 
@@ -15,20 +21,26 @@
 
 namespace npbcg_detail {
 
+constexpr int kSliceRows = 64;
+
 template <class V>
-struct TiledMatrix {
-    int n = 0, T = 0, W = 0, nblk = 0, ntile = 0;
-    long long nnz = 0;
-    std::vector<int> seg;        // ntile * nblk * (T+1)
-    std::vector<V> val;          // nnz
-    std::vector<uint16_t> col;   // nnz
+struct SellBlocks {
+    int n = 0, W = 0, nblk = 0, R = 0, nrange = 0;
+    long long nnz = 0;     // entries of the CSR
+    long long stored = 0;  // entries stored, padding included
+    std::vector<int> wg_slice;   // nblk * nrange + 1
+    std::vector<int> slice_off;  // slices + 1
+    std::vector<int> slice_row;  // slices * 64
+    std::vector<V> val;          // stored
+    std::vector<uint16_t> col;   // stored
 };
 
-// Builds the tiled copy.  Returns false if W > 65536 or the sizes overflow.
-bool build_tiled(int n, const int *rowstr, const int *colidx, const double *a,
-                 int T, int W, TiledMatrix<double> *out);
-bool build_tiled(int n, const int *rowstr, const int *colidx, const double *a,
-                 int T, int W, TiledMatrix<float> *out);
+// Builds the copy; the CSR must have ascending columns within each row.
+// Returns false if W > 65536 or the sizes overflow 32-bit offsets.
+bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
+                       int W, int R, SellBlocks<double> *out);
+bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
+                       int W, int R, SellBlocks<float> *out);
 
 }  // namespace npbcg_detail
 

@@ -84,6 +84,7 @@ def _load() -> ctypes.CDLL:
         "npbcg_set_spmv_sampling": (ctypes.c_int, [vp, ctypes.c_int]),
         "npbcg_get_timing": (ctypes.c_int, [vp, ctypes.POINTER(Timing)]),
         "npbcg_spmv_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
+        "npbcg_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
         "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
         "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
                                           _F64P, _F64P, _DP]),
@@ -238,6 +239,13 @@ class Context:
         t = Timing()
         _check(LIB.npbcg_get_timing(self._h, ctypes.byref(t)), "npbcg_get_timing")
         return t
+
+    def describe(self) -> dict:
+        import json
+
+        buf = ctypes.create_string_buffer(1024)
+        _check(LIB.npbcg_describe(self._h, buf, 1024), "npbcg_describe")
+        return json.loads(buf.value.decode())
 
     def spmv_bench(self, reps: int = 20) -> float:
         ms = ctypes.c_double(0.0)

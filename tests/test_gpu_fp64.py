@@ -130,6 +130,7 @@ def test_tiled_layout_shapes(gpu, golden, tile_rows, tile_width):
     x[:n] += np.linspace(-0.3, 0.3, n)
     with gpu.Context(m, spmv_variant=5, tile_rows=tile_rows, tile_width=tile_width) as ctx:
         z, p, q, r, rn = ctx.conj_grad(x)
+        ctx.fill_x(1.0)  # the drop-in call left x on the device
         zeta, _ = ctx.power(4, c["shift"])
     oz, op, oq, orr, orn = oracle.conj_grad(_ocsr(m), x)
     for v, ref, lab in [(z, oz, "z"), (q, oq, "q"), (r, orr, "r")]:
