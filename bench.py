@@ -120,6 +120,71 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+PROFILE_COPY_BYTES = 256 << 20
+
+
+def profile_pass():
+    """Short run for hardware-counter passes: a copy of known size (the
+    calibration kernel), then two outer iterations launched without graphs."""
+    import npbcg
+
+    wl = WORKLOAD
+    m = npbcg.makea(wl["n"], wl["nonzer"], wl["shift"], wl["seed"], wl["rcond"])
+    with npbcg.Context(m, device=0, cgitmax=wl["cgitmax"], use_graph=0) as ctx:
+        ctx.copy_bench(PROFILE_COPY_BYTES, 3)
+        ctx.fill_x(1.0)
+        ctx.power(2, wl["shift"])
+        print(json.dumps({"profile_pass": True, "copy_bytes": PROFILE_COPY_BYTES,
+                          "kernel_config": ctx.describe()}))
+
+
+def _pmc_rows(path, counter):
+    import csv
+
+    out = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name", counter) != counter:
+                continue
+            name = row.get("Kernel_Name", "")
+            val = float(row.get("Counter_Value", "nan"))
+            out.setdefault(name, []).append(val)
+    return out
+
+
+def pmc_summary(fetch_csv, write_csv, out_path):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the SpMV kernel, scaled by
+    what the counters read for the copy kernel of known traffic."""
+    fetch = _pmc_rows(fetch_csv, "FETCH_SIZE")
+    write = _pmc_rows(write_csv, "WRITE_SIZE")
+
+    def pick(d, key):
+        ks = [k for k in d if key in k]
+        return (ks[0], d[ks[0]]) if ks else (None, [])
+
+    _, cf = pick(fetch, "copy_kernel")
+    _, cw = pick(write, "copy_kernel")
+    kname, sf = pick(fetch, "spmv_sell_kernel")
+    if kname is None:
+        kname, sf = pick(fetch, "spmv_csr_kernel")
+    _, sw = pick(write, kname.split("<")[0] if kname else "spmv")
+    expect_kib = PROFILE_COPY_BYTES / 1024.0
+    f_scale = float(np.median(cf)) / expect_kib
+    w_scale = float(np.median(cw)) / expect_kib
+    spmv_f = float(np.median(sf)) * 1024.0 / f_scale
+    spmv_w = float(np.median(sw)) * 1024.0 / w_scale
+    doc = {"kernel": kname, "dispatches": len(sf),
+           "copy_fetch_kib_per_dispatch": float(np.median(cf)),
+           "copy_write_kib_per_dispatch": float(np.median(cw)),
+           "copy_bytes_each_way": PROFILE_COPY_BYTES,
+           "fetch_scale": f_scale, "write_scale": w_scale,
+           "spmv_fetch_bytes": spmv_f, "spmv_write_bytes": spmv_w,
+           "bytes_per_launch": spmv_f + spmv_w}
+    with open(out_path, "w") as f:
+        json.dump(doc, f, indent=1)
+    print(json.dumps(doc))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,7 +194,18 @@ def main():
                     help="outer iterations in the CPU baseline sample (0 = skip)")
     ap.add_argument("--spmv-variant", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--fuse-p-update", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="short run for rocprofv3 counter passes")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="summarise two counter passes into the traffic file")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
+    if args.profile_pass:
+        profile_pass()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the path is not sharded "
@@ -145,7 +221,8 @@ def main():
     m = npbcg.makea(wl["n"], wl["nonzer"], wl["shift"], wl["seed"], wl["rcond"])
     t_gen = time.perf_counter() - t0
     ctx = npbcg.Context(m, device=0, cgitmax=wl["cgitmax"], spmv_variant=args.spmv_variant,
-                        use_graph=0 if args.no_graph else 1)
+                        use_graph=0 if args.no_graph else 1,
+                        fuse_p_update=1 if args.fuse_p_update else 0)
     n, nnz = m.n, m.nnz
 
     for _ in range(args.warmup):
@@ -193,6 +270,15 @@ def main():
         "samples": int(sum(1 for _ in spmv_ms)) * 25,
     }
 
+    # Streaming reference: a device copy at the matrix's size (cache-resident
+    # like the matrix) and at 1 GiB (HBM).
+    copy_small = ctx.copy_bench(b_spmv // 2 // 16 * 16, 10)
+    copy_big = ctx.copy_bench(1 << 30, 5)
+    stream = {"copy_gbs_at_matrix_size": round(2 * (b_spmv // 2 // 16 * 16) / copy_small / 1e6, 1),
+              "copy_gbs_1GiB": round(2 * (1 << 30) / copy_big / 1e6, 1)}
+    roofline["frac_of_copy_at_matrix_size"] = round(
+        achieved / stream["copy_gbs_at_matrix_size"], 4)
+
     cpu = None
     if rank == 0 and args.cpu_outer_iters > 0:
         cpu = cpu_baseline(m, wl, args.cpu_outer_iters)
@@ -220,6 +306,7 @@ def main():
         "generate_seconds": round(t_gen, 2),
         "kernel_config": ctx.describe(),
         "roofline": roofline,
+        "stream_copy": stream,
         "cpu_baseline": cpu,
     }
     ctx.close()

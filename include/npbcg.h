@@ -52,7 +52,9 @@ typedef struct npbcg_opts {
     int use_graph;    /* nonzero: replay each outer iteration as a captured HIP graph */
     int tile_rows;    /* LDS-staged SpMV: rows per tile, 0 = auto */
     int tile_width;   /* LDS-staged SpMV: vector entries per LDS block, 0 = auto */
-    int reserved[9];
+    int fuse_p_update;/* LDS-staged SpMV: form p = r + beta p inside the next SpMV
+                         instead of a separate launch (default 0) */
+    int reserved[8];
 } npbcg_opts;
 
 /* Timing of the last npbcg_power / npbcg_conj_grad_device call, from HIP
@@ -155,6 +157,14 @@ int npbcg_describe(const npbcg_ctx *ctx, char *buf, int len);
  * per-block row sums), repeated reps times; *avg_ms gets the mean duration
  * from HIP events.  Used to measure SpMV bandwidth in isolation. */
 int npbcg_spmv_bench(npbcg_ctx *ctx, int reps, double *avg_ms);
+
+/* A plain device-to-device copy of `bytes` (16-byte vector loads and
+ * stores, grid-stride) repeated reps times on the context's device;
+ * *avg_ms is the mean duration.  It has no counterpart in the reference: it
+ * measures the achievable streaming bandwidth the SpMV is compared with, and
+ * calibrates memory-traffic counters with a kernel of known traffic.  The
+ * two buffers are allocated and freed inside the call. */
+int npbcg_copy_bench(npbcg_ctx *ctx, long long bytes, int reps, double *avg_ms);
 
 /* ---- reference-shaped shim ------------------------------------------------
  * For a caller that keeps the reference's exact call

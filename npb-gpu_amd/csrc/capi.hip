@@ -64,7 +64,7 @@ struct npbcg_ctx {
     double *d_rows_part = nullptr; // nblk * n per-block row sums
     bool cols_sorted = true;
     long long sell_stored = 0;     // SELL entries, padding included
-    double *d_vec = nullptr;   // x, z, p, q, r back to back, stride vstride
+    double *d_vec = nullptr;   // x, z, p, q, r, p_alt back to back, stride vstride
     size_t vstride = 0;
     double *d_part = nullptr;  // all partial-sum buffers, carved by refresh_problem
     int part_vec = 0, part_spmv = 0;
@@ -120,6 +120,7 @@ void refresh_problem(npbcg_ctx *c) {
     pb.p = vec_ptr(c, NPBCG_VEC_P);
     pb.q = vec_ptr(c, NPBCG_VEC_Q);
     pb.r = vec_ptr(c, NPBCG_VEC_R);
+    pb.p_alt = vec_ptr(c, NPBCG_VEC_R + 1);
     double *pp = c->d_part;
     const size_t nv = c->part_vec, ns = c->part_spmv;
     pb.part.xx = pp;
@@ -407,7 +408,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     CREATE_HIP(hipMalloc(&c->d_rowstr, sizeof(int) * ((size_t)n + 1)));
     CREATE_HIP(hipMalloc(&c->d_colidx, sizeof(int) * std::max<long long>(1, nnz)));
     CREATE_HIP(hipMalloc(&c->d_a, sizeof(double) * std::max<long long>(1, nnz)));
-    CREATE_HIP(hipMalloc(&c->d_vec, sizeof(double) * 5 * c->vstride));
+    CREATE_HIP(hipMalloc(&c->d_vec, sizeof(double) * 6 * c->vstride));
     c->part_vec = vec_blocks;
     c->part_spmv = variant == npbcg_dev::kSpmvTiled ? std::max(sd.grid, vec_blocks)
                                                     : spmv_blocks;
@@ -454,13 +455,14 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
         CREATE_HIP(hipMemcpyAsync(c->d_a, a, sizeof(double) * nnz, hipMemcpyHostToDevice,
                                   c->stream));
     }
-    CREATE_HIP(hipMemsetAsync(c->d_vec, 0, sizeof(double) * 5 * c->vstride, c->stream));
+    CREATE_HIP(hipMemsetAsync(c->d_vec, 0, sizeof(double) * 6 * c->vstride, c->stream));
     CREATE_HIP(hipMemsetAsync(c->d_sc, 0, npbcg_dev::scalars_bytes(), c->stream));
 
     refresh_problem(c);
     c->pb.vec_blocks = vec_blocks;
     c->pb.spmv_blocks = spmv_blocks;
     c->pb.spmv_variant = variant;
+    c->pb.fuse_p_update = c->opts.fuse_p_update ? 1 : 0;
     c->pb.sell = sd;
     CREATE_HIP(npbcg_dev::launch_fill(c->pb.x, n + 1, 1.0, vec_blocks, c->stream));
     CREATE_HIP(hipStreamSynchronize(c->stream));
@@ -699,10 +701,10 @@ int npbcg_describe(const npbcg_ctx *c, char *buf, int len) {
         "\"csr_blocks\": %d, \"block_width\": %d, \"col_blocks\": %d, "
         "\"range_rows\": %d, \"row_ranges\": %d, \"sell_workgroups\": %d, "
         "\"sell_threads\": %d, \"sell_stored\": %lld, \"cgitmax\": %d, "
-        "\"use_graph\": %d}",
+        "\"use_graph\": %d, \"fuse_p_update\": %d}",
         c->n, c->nnz, c->pb.spmv_variant, c->pb.vec_blocks, c->pb.spmv_blocks, sd.W,
         sd.nblk, sd.R, sd.nrange, sd.grid, sd.block, c->sell_stored, c->cgitmax,
-        c->opts.use_graph);
+        c->opts.use_graph, c->pb.fuse_p_update);
     if (len > 0) {
         std::strncpy(buf, tmp, (size_t)len - 1);
         buf[len - 1] = '\0';
@@ -721,6 +723,34 @@ int npbcg_spmv_bench(npbcg_ctx *c, int reps, double *avg_ms) {
     NPBCG_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     NPBCG_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    *avg_ms = ms / reps;
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_copy_bench(npbcg_ctx *c, long long bytes, int reps, double *avg_ms) {
+    if (!c || bytes < 16 || reps < 1 || !avg_ms)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_copy_bench: invalid argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    bytes &= ~15LL;
+    void *src = nullptr, *dst = nullptr;
+    hipError_t e = hipMalloc(&src, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(src, 0, (size_t)bytes, c->stream);
+    const int blocks = 8 * 256;
+    if (e == hipSuccess) e = npbcg_dev::launch_copy(src, dst, bytes, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_start, c->stream);
+    for (int i = 0; i < reps && e == hipSuccess; i++)
+        e = npbcg_dev::launch_copy(src, dst, bytes, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_stop, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess)
+        return set_error(e == hipErrorOutOfMemory ? NPBCG_ERR_NOMEM : NPBCG_ERR_DEVICE,
+                         std::string("npbcg_copy_bench: ") + hipGetErrorString(e));
     *avg_ms = ms / reps;
     clear_error();
     return NPBCG_OK;

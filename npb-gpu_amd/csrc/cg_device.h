@@ -81,21 +81,26 @@ __device__ __forceinline__ void store_block_partial(double acc, double *part) {
     }
 }
 
-// Sum of part[0..count) by the whole workgroup, in a fixed order that depends
-// only on count and BLOCK; every thread receives the result.
+// Sum of part[0..count) by the whole workgroup, in a fixed order that
+// depends only on count: the first kBlock threads sum strided slices, their
+// four wavefront sums are added in order.  Any BLOCK >= kBlock therefore gets
+// the same bits, so kernels of different shapes agree on alpha and beta.
+// Every thread receives the result.
 template <int BLOCK>
 __device__ __forceinline__ double reduce_partials(const double *__restrict__ part, int count) {
-    constexpr int kWaves = BLOCK / kWave;
-    __shared__ double wsum[kWaves];
+    static_assert(BLOCK % kBlock == 0, "BLOCK must be a multiple of kBlock");
+    __shared__ double wsum[kWavesPerBlock];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < count; i += BLOCK) s += part[i];
-    s = segment_sum<64>(s);
-    if (lane == 0) wsum[wid] = s;
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x / kWave) < kWavesPerBlock) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < count; i += kBlock) s += part[i];
+        s = segment_sum<64>(s);
+        if (lane == 0) wsum[wid] = s;
+    }
     __syncthreads();
     double t = wsum[0];
-    for (int w = 1; w < kWaves; w++) t += wsum[w];
+    for (int w = 1; w < kWavesPerBlock; w++) t += wsum[w];
     __syncthreads();  // wsum may be reused by the caller's next reduction
     return t;
 }

@@ -19,6 +19,8 @@
 // no grid-wide fences or atomics: each workgroup stores one partial and the
 // next launch reduces all partials in a fixed order (every consumer
 // workgroup computes the same bits).  Kernel boundaries give the ordering.
+#include <hip/hip_ext.h>
+
 #include "cg_device.h"
 #include "cg_kernels.h"
 
@@ -113,13 +115,17 @@ __global__ __launch_bounds__(kBlock) void spmv_csr_kernel(
 // partial of p.q = sum_b sum_rows p[row] * part[b][row].
 // ---------------------------------------------------------------------------
 
-template <int BLOCK, int UNR, int MODE>
+// PUPD: the input vector is p_new = r + beta p_old, built here instead of by
+// a separate update_p launch.  beta comes from the r.r partials (rr_part new,
+// rho_part old); the workgroups of row range 0 store their block of p_new.
+template <int BLOCK, int UNR, int MODE, bool PUPD>
 __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     const int *__restrict__ wg_slice, const int *__restrict__ slice_off,
     const int *__restrict__ slice_row, const double *__restrict__ val,
     const uint16_t *__restrict__ col, const double *__restrict__ v,
     double *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W,
-    int nrange) {
+    int nrange, const double *__restrict__ r, double *__restrict__ p_new,
+    const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec) {
     // Dynamic LDS: W vector entries, then one p.q slot per slice of this
     // workgroup (the p.q partial is summed in slice order).
     extern __shared__ double2 lds_v2[];  // 16-byte aligned
@@ -130,7 +136,29 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     const int b = wg / nrange;
     const int c0 = b * W;
     const int wb = min(W, n - c0);
-    {
+    double beta = 0.0;
+    if constexpr (PUPD)
+        beta = reduce_partials<BLOCK>(rr_part, n_vec) / reduce_partials<BLOCK>(rho_part, n_vec);
+    if constexpr (PUPD) {
+        const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
+        const double2 *r2 = reinterpret_cast<const double2 *>(r + c0);
+        double2 *o2 = reinterpret_cast<double2 *>(p_new + c0);
+        const bool store = (wg % nrange) == 0;
+#pragma unroll 4
+        for (int i = threadIdx.x; i < wb / 2; i += BLOCK) {
+            const double2 pv = v2[i], rv = r2[i];
+            double2 nv;
+            nv.x = rv.x + beta * pv.x;
+            nv.y = rv.y + beta * pv.y;
+            lds_v2[i] = nv;
+            if (store) o2[i] = nv;
+        }
+        if ((wb & 1) && threadIdx.x == 0) {
+            const double nv = r[c0 + wb - 1] + beta * v[c0 + wb - 1];
+            vblk[wb - 1] = nv;
+            if (store) p_new[c0 + wb - 1] = nv;
+        }
+    } else {
         const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
 #pragma unroll 4
         for (int i = threadIdx.x; i < wb / 2; i += BLOCK) lds_v2[i] = v2[i];
@@ -173,7 +201,9 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
         const int row = slice_row[(size_t)s * kWave + lane];
         if (row >= 0) part_b[row] = acc;
         if constexpr (MODE == 0) {
-            const double d = segment_sum<64>(row >= 0 ? v[row] * acc : 0.0);
+            double pr = 0.0;
+            if (row >= 0) pr = PUPD ? r[row] + beta * v[row] : v[row];
+            const double d = segment_sum<64>(pr * acc);
             if (lane == 0) slice_dot[k] = d;
         }
     }
@@ -258,9 +288,10 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     store_block_partial<kBlock>(acc, rr_out);
 }
 
-// p = r + beta p with beta = (new r.r) / (old r.r), both from partials.
+// p_out = r + beta p_in with beta = (new r.r) / (old r.r), both from
+// partials (p_in may equal p_out).
 __global__ __launch_bounds__(kBlock) void update_p_kernel(
-    const double *__restrict__ r, double *__restrict__ p, int n,
+    const double *__restrict__ r, const double *p_in, double *p_out, int n,
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
     Scalars *sc) {
     const double rho = reduce_partials<kBlock>(rr_part, n_vec);
@@ -271,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void update_p_kernel(
         sc->beta = beta;
     }
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
-        p[j] = r[j] + beta * p[j];
+        p_out[j] = r[j] + beta * p_in[j];
 }
 
 // Start of conj_grad (cg.cu:671-686): q = z = 0, r = p = x for j <= n and
@@ -362,6 +393,14 @@ __global__ __launch_bounds__(kBlock) void scale_x_kernel(
         x[j] = s * z[j];
 }
 
+// dst = src over `count` 16-byte elements (bandwidth reference).
+__global__ __launch_bounds__(kBlock) void copy_kernel(const double2 *__restrict__ src,
+                                                     double2 *__restrict__ dst, long long count) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < count;
+         i += (long long)gridDim.x * kBlock)
+        dst[i] = src[i];
+}
+
 // Resets the outer-loop bookkeeping.  A kernel rather than a memset so it may
 // sit in a captured graph.
 __global__ void reset_scalars_kernel(Scalars *sc, double shift) {
@@ -382,50 +421,84 @@ __global__ void fill_kernel(double *__restrict__ v, int count, double value) {
 
 namespace {
 
+// Kernel launch on `st`; with events, they are recorded by the dispatch
+// itself (hipExtLaunchKernelGGL), so their interval is the kernel's own run.
+template <typename K, typename... Args>
+hipError_t launch_k(K kernel, int grid, int block, int lds, hipStream_t st, hipEvent_t e0,
+                    hipEvent_t e1, Args... args) {
+    if (e0)
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (uint32_t)lds, st, e0, e1, 0u,
+                              args...);
+    else
+        kernel<<<grid, block, lds, st>>>(args...);
+    return hipGetLastError();
+}
+
 template <int SEG, int UNR, int MODE>
-hipError_t launch_csr_t(const Fp64Problem &pb, hipStream_t st) {
+hipError_t launch_csr_t(const Fp64Problem &pb, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const double *v = MODE == 0 ? pb.p : pb.z;
     double *y = MODE == 0 ? pb.q : pb.r;
     double *part = MODE == 0 ? pb.part.pq : pb.part.res;
-    spmv_csr_kernel<SEG, UNR, MODE><<<pb.spmv_blocks, kBlock, 0, st>>>(
-        pb.rowstr, pb.colidx, pb.a, v, y, pb.x, pb.n, part);
-    return hipGetLastError();
+    return launch_k(spmv_csr_kernel<SEG, UNR, MODE>, pb.spmv_blocks, kBlock, 0, st, e0, e1,
+                    pb.rowstr, pb.colidx, pb.a, v, y, pb.x, pb.n, part);
 }
 
-template <int BLOCK, int MODE>
-hipError_t launch_sell_t(const Fp64Problem &pb, hipStream_t st) {
-    const SellDev &d = pb.sell;
-    const double *v = MODE == 0 ? pb.p : pb.z;
-    spmv_sell_kernel<BLOCK, 8, MODE><<<d.grid, BLOCK, d.lds_bytes, st>>>(
-        d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v, d.rows_part, pb.part.pq, pb.n,
-        d.W, d.nrange);
-    return hipGetLastError();
+bool fused_p(const Fp64Problem &pb) {
+    return pb.spmv_variant == kSpmvTiled && pb.fuse_p_update;
 }
 
-template <int MODE>
-hipError_t launch_spmv(const Fp64Problem &pb, hipStream_t st) {
-    switch (pb.spmv_variant) {
-    case kSpmvTiled:
-        return pb.sell.block == 512 ? launch_sell_t<512, MODE>(pb, st)
-                                    : launch_sell_t<1024, MODE>(pb, st);
-    case kSpmvSeg16: return launch_csr_t<16, 4, MODE>(pb, st);
-    case kSpmvSeg32: return launch_csr_t<32, 4, MODE>(pb, st);
-    case kSpmvSeg64U2: return launch_csr_t<64, 2, MODE>(pb, st);
-    default: return launch_csr_t<64, 4, MODE>(pb, st);
-    }
-}
-
-template <int BLOCK, int MODE>
-hipError_t set_sell_lds_t(int bytes) {
-    return hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&spmv_sell_kernel<BLOCK, 8, MODE>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+// The p buffer holding CG step `step`'s direction on the SELL path, where p
+// alternates between p and p_alt (the fused update writes the other one).
+double *p_of_step(const Fp64Problem &pb, int step) {
+    return (fused_p(pb) && (step & 1)) ? pb.p_alt : pb.p;
 }
 
 // The rho partials feeding CG step `step`: x.x from init for step 0, else
 // the r.r written by the previous step's update_zr.
 const double *rho_source(const Fp64Problem &pb, int step) {
     return step == 0 ? pb.part.xx : pb.part.rr[(step - 1) & 1];
+}
+
+// MODE 0 at `step` (q = A p): for step > 0 the kernel forms p of this step
+// from r and the previous step's p.  MODE 1: r = A z.
+template <int BLOCK, int MODE>
+hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEvent_t e0,
+                         hipEvent_t e1) {
+    const SellDev &d = pb.sell;
+    if (MODE == 0 && step > 0 && fused_p(pb)) {
+        return launch_k(spmv_sell_kernel<BLOCK, 8, 0, true>, d.grid, BLOCK, d.lds_bytes, st,
+                        e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col,
+                        (const double *)p_of_step(pb, step - 1), d.rows_part, pb.part.pq,
+                        pb.n, d.W, d.nrange, (const double *)pb.r, p_of_step(pb, step),
+                        (const double *)pb.part.rr[(step - 1) & 1],
+                        rho_source(pb, step - 1), pb.vec_blocks);
+    }
+    const double *v = MODE == 0 ? p_of_step(pb, step) : pb.z;
+    return launch_k(spmv_sell_kernel<BLOCK, 8, MODE, false>, d.grid, BLOCK, d.lds_bytes, st,
+                    e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v, d.rows_part,
+                    pb.part.pq, pb.n, d.W, d.nrange, (const double *)nullptr,
+                    (double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0);
+}
+
+template <int MODE>
+hipError_t launch_spmv(const Fp64Problem &pb, hipStream_t st, hipEvent_t e0 = nullptr,
+                       hipEvent_t e1 = nullptr, int step = 0) {
+    switch (pb.spmv_variant) {
+    case kSpmvTiled:
+        return pb.sell.block == 512 ? launch_sell_t<512, MODE>(pb, step, st, e0, e1)
+                                    : launch_sell_t<1024, MODE>(pb, step, st, e0, e1);
+    case kSpmvSeg16: return launch_csr_t<16, 4, MODE>(pb, st, e0, e1);
+    case kSpmvSeg32: return launch_csr_t<32, 4, MODE>(pb, st, e0, e1);
+    case kSpmvSeg64U2: return launch_csr_t<64, 2, MODE>(pb, st, e0, e1);
+    default: return launch_csr_t<64, 4, MODE>(pb, st, e0, e1);
+    }
+}
+
+template <int BLOCK, int MODE, bool PUPD>
+hipError_t set_sell_lds_t(int bytes) {
+    return hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&spmv_sell_kernel<BLOCK, 8, MODE, PUPD>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 }  // namespace
@@ -440,8 +513,12 @@ int resid_partials(const Fp64Problem &pb) {
 
 hipError_t prepare_sell_kernels(int lds_bytes) {
     hipError_t e = hipSuccess;
-    hipError_t r[4] = {set_sell_lds_t<512, 0>(lds_bytes), set_sell_lds_t<1024, 0>(lds_bytes),
-                       set_sell_lds_t<512, 1>(lds_bytes), set_sell_lds_t<1024, 1>(lds_bytes)};
+    hipError_t r[6] = {set_sell_lds_t<512, 0, false>(lds_bytes),
+                       set_sell_lds_t<1024, 0, false>(lds_bytes),
+                       set_sell_lds_t<512, 0, true>(lds_bytes),
+                       set_sell_lds_t<1024, 0, true>(lds_bytes),
+                       set_sell_lds_t<512, 1, false>(lds_bytes),
+                       set_sell_lds_t<1024, 1, false>(lds_bytes)};
     for (hipError_t x : r)
         if (x != hipSuccess) e = x;
     return e;
@@ -490,16 +567,18 @@ hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st) {
 
 hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st) {
     update_zr_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
-        pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, step), pb.vec_blocks, pb.part.pq,
+        p_of_step(pb, step), pb.q, pb.z, pb.r, pb.n, rho_source(pb, step), pb.vec_blocks,
+        pb.part.pq,
         spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part,
         pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc);
     return hipGetLastError();
 }
 
+// p = r + beta p closing CG step `step`; the result always lands in pb.p.
 hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st) {
-    update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.r, pb.p, pb.n, pb.part.rr[step & 1],
-                                                      rho_source(pb, step), pb.vec_blocks,
-                                                      pb.sc);
+    update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+        pb.r, p_of_step(pb, step), pb.p, pb.n, pb.part.rr[step & 1], rho_source(pb, step),
+        pb.vec_blocks, pb.sc);
     return hipGetLastError();
 }
 
@@ -523,6 +602,13 @@ hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t
     return hipGetLastError();
 }
 
+hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
+                       hipStream_t st) {
+    copy_kernel<<<blocks, kBlock, 0, st>>>(static_cast<const double2 *>(src),
+                                           static_cast<double2 *>(dst), bytes / 16);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream_t st) {
     fill_kernel<<<blocks, kBlock, 0, st>>>(v, count, value);
     return hipGetLastError();
@@ -530,14 +616,16 @@ hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream
 
 hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, int record,
                              hipStream_t st, const SpmvProbe *probe) {
+    // With the fused SELL path, p = r + beta p of step it-1 is formed inside
+    // step it's SpMV, so a step is two launches; one update_p closes the last.
+    const bool fused = fused_p(pb);
     hipError_t e = launch_cg_init(pb, scale_x, st);
     for (int it = 0; it < cgitmax && e == hipSuccess; it++) {
         const bool timed = probe && probe->start && it < probe->count;
-        if (timed) e = hipEventRecord(probe->start[it], st);
-        if (e == hipSuccess) e = launch_spmv_pq(pb, st);
-        if (timed && e == hipSuccess) e = hipEventRecord(probe->stop[it], st);
+        e = launch_spmv<0>(pb, st, timed ? probe->start[it] : nullptr,
+                           timed ? probe->stop[it] : nullptr, it);
         if (e == hipSuccess) e = launch_update_zr(pb, it, st);
-        if (e == hipSuccess) e = launch_update_p(pb, it, st);
+        if (e == hipSuccess && (!fused || it == cgitmax - 1)) e = launch_update_p(pb, it, st);
     }
     if (e == hipSuccess) e = launch_resid(pb, record, st);
     return e;

@@ -52,6 +52,7 @@ struct Fp64Problem {
     const int *colidx;
     const double *a;
     double *x, *z, *p, *q, *r;  // n+2 entries each
+    double *p_alt;              // second direction buffer (SELL path ping-pong)
     Partials part;
     Scalars *sc;
     double *zeta_hist, *rnorm_hist;
@@ -59,6 +60,7 @@ struct Fp64Problem {
     int vec_blocks;             // workgroups of the element-wise kernels
     int spmv_blocks;            // workgroups of the CSR SpMV kernels
     int spmv_variant;
+    int fuse_p_update;          // SELL path: p update folded into the next SpMV
     SellDev sell;
 };
 
@@ -95,6 +97,9 @@ hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_scalars,
                           hipStream_t st);
 hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st);
 hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream_t st);
+// Copy of bytes (a multiple of 16) from src to dst.
+hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
+                       hipStream_t st);
 
 // One conj_grad: init (optionally applying x = xscale z first), cgitmax x
 // (spmv, update_zr, update_p), residual SpMV and finish.

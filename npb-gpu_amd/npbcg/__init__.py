@@ -46,7 +46,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("cgitmax", ctypes.c_int),
                 ("engine", ctypes.c_int), ("spmv_variant", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("tile_rows", ctypes.c_int),
-                ("tile_width", ctypes.c_int), ("reserved", ctypes.c_int * 9)]
+                ("tile_width", ctypes.c_int), ("fuse_p_update", ctypes.c_int),
+                ("reserved", ctypes.c_int * 8)]
 
 
 class Timing(ctypes.Structure):
@@ -85,6 +86,7 @@ def _load() -> ctypes.CDLL:
         "npbcg_get_timing": (ctypes.c_int, [vp, ctypes.POINTER(Timing)]),
         "npbcg_spmv_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
         "npbcg_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
+        "npbcg_copy_bench": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int, _DP]),
         "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
         "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
                                           _F64P, _F64P, _DP]),
@@ -246,6 +248,13 @@ class Context:
         buf = ctypes.create_string_buffer(1024)
         _check(LIB.npbcg_describe(self._h, buf, 1024), "npbcg_describe")
         return json.loads(buf.value.decode())
+
+    def copy_bench(self, nbytes: int, reps: int = 10) -> float:
+        """Mean ms of a device copy of nbytes (read nbytes + write nbytes)."""
+        ms = ctypes.c_double(0.0)
+        _check(LIB.npbcg_copy_bench(self._h, nbytes, reps, ctypes.byref(ms)),
+               "npbcg_copy_bench")
+        return ms.value
 
     def spmv_bench(self, reps: int = 20) -> float:
         ms = ctypes.c_double(0.0)

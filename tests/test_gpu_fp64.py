@@ -128,7 +128,8 @@ def test_tiled_layout_shapes(gpu, golden, tile_rows, tile_width):
     n = m.n
     x = np.ones(n + 2)
     x[:n] += np.linspace(-0.3, 0.3, n)
-    with gpu.Context(m, spmv_variant=5, tile_rows=tile_rows, tile_width=tile_width) as ctx:
+    with gpu.Context(m, spmv_variant=5, tile_rows=tile_rows, tile_width=tile_width,
+                     fuse_p_update=tile_rows % 2) as ctx:
         z, p, q, r, rn = ctx.conj_grad(x)
         ctx.fill_x(1.0)  # the drop-in call left x on the device
         zeta, _ = ctx.power(4, c["shift"])
@@ -139,11 +140,11 @@ def test_tiled_layout_shapes(gpu, golden, tile_rows, tile_width):
     _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"][:4]], "tiled power")
 
 
-@pytest.mark.parametrize("variant", [1, 5])
-def test_class_b_variants_match_reference(gpu, golden, variant):
+@pytest.mark.parametrize("variant,fuse", [(1, 0), (5, 0), (5, 1)])
+def test_class_b_variants_match_reference(gpu, golden, variant, fuse):
     c = golden["B_npb"]
     m = _gen(gpu, c)
-    with gpu.Context(m, spmv_variant=variant) as ctx:
+    with gpu.Context(m, spmv_variant=variant, fuse_p_update=fuse) as ctx:
         zeta, _ = ctx.power(c["niter"], c["shift"])
     _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"]], f"B variant {variant}")
 
