@@ -185,6 +185,51 @@ def pmc_summary(fetch_csv, write_csv, out_path):
     print(json.dumps(doc))
 
 
+SOLVE_WORKLOAD = dict(workload="npb_cg_solve_n150000", n=150000, nonzer=15, shift=110.0,
+                      rcond=0.1, seed=314.0, rtol=1e-10)
+
+
+def solve_comparison(reps: int):
+    """Time to a 1e-10 FP64-checked relative residual on the n=150000 matrix
+    (b = 2u - 1 from the LCG stream seeded 315): plain FP64 CG vs FP64-refined
+    mixed precision, each the median of `reps` solves with b resident."""
+    import npbcg
+
+    sw = SOLVE_WORKLOAD
+    m = npbcg.makea(sw["n"], sw["nonzer"], sw["shift"], sw["seed"], sw["rcond"])
+    b = rhs_for(sw["n"], sw["seed"])
+    out = {"workload": sw["workload"], "n": sw["n"], "nnz": m.nnz, "rtol": sw["rtol"]}
+    with npbcg.Context(m, device=0) as ctx:
+        ctx.set_rhs(b)
+        for name, mode in (("fp64", npbcg.SOLVE_FP64), ("mixed", npbcg.SOLVE_MIXED)):
+            ctx.solve(mode, sw["rtol"], 500)  # warm
+            ev, wall, st = [], [], None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                st = ctx.solve(mode, sw["rtol"], 500)
+                wall.append((time.perf_counter() - t0) * 1e3)
+                ev.append(st.time_ms)
+            d = st.as_dict()
+            d["time_ms_median"] = round(float(np.median(ev)), 4)
+            d["wall_ms_median"] = round(float(np.median(wall)), 4)
+            out[name] = d
+    out["mixed_speedup_vs_fp64"] = round(out["fp64"]["time_ms_median"] /
+                                         out["mixed"]["time_ms_median"], 3)
+    return out
+
+
+def rhs_for(n: int, seed: float) -> np.ndarray:
+    """b_i = 2 u_i - 1, u_i successive draws of the 46-bit LCG (a = 5^13)
+    from seed + 1 (the definition the oracle's orc_make_rhs restates)."""
+    x = int(seed) + 1
+    a, mask = 1220703125, (1 << 46) - 1
+    out = np.empty(n)
+    for i in range(n):
+        x = (a * x) & mask
+        out[i] = 2.0 * (x / float(1 << 46)) - 1.0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +240,8 @@ def main():
     ap.add_argument("--spmv-variant", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fuse-p-update", action="store_true")
+    ap.add_argument("--solve-reps", type=int, default=10,
+                    help="solves per mode in the n=150000 time-to-1e-10 comparison (0 = skip)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short run for rocprofv3 counter passes")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
@@ -279,6 +326,8 @@ def main():
     roofline["frac_of_copy_at_matrix_size"] = round(
         achieved / stream["copy_gbs_at_matrix_size"], 4)
 
+    solve = solve_comparison(args.solve_reps) if args.solve_reps > 0 else None
+
     cpu = None
     if rank == 0 and args.cpu_outer_iters > 0:
         cpu = cpu_baseline(m, wl, args.cpu_outer_iters)
@@ -307,6 +356,7 @@ def main():
         "kernel_config": ctx.describe(),
         "roofline": roofline,
         "stream_copy": stream,
+        "solve_to_1e-10": solve,
         "cpu_baseline": cpu,
     }
     ctx.close()

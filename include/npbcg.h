@@ -54,7 +54,9 @@ typedef struct npbcg_opts {
     int tile_width;   /* LDS-staged SpMV: vector entries per LDS block, 0 = auto */
     int fuse_p_update;/* LDS-staged SpMV: form p = r + beta p inside the next SpMV
                          instead of a separate launch (default 0) */
-    int reserved[8];
+    int solve_batch;  /* npbcg_solve: CG steps launched before the first host check
+                         (later checks use half); 0 -> 10 */
+    int reserved[7];
 } npbcg_opts;
 
 /* Timing of the last npbcg_power / npbcg_conj_grad_device call, from HIP
@@ -165,6 +167,43 @@ int npbcg_spmv_bench(npbcg_ctx *ctx, int reps, double *avg_ms);
  * calibrates memory-traffic counters with a kernel of known traffic.  The
  * two buffers are allocated and freed inside the call. */
 int npbcg_copy_bench(npbcg_ctx *ctx, long long bytes, int reps, double *avg_ms);
+
+/* ---- solve to a tolerance ---------------------------------------------------
+ * A x = b from x = 0 until the FP64 true relative residual ||b - A x|| / ||b||
+ * (recomputed with the FP64 matrix) is <= rtol.  The reference has no such
+ * entry: it always runs 25 steps (cg.cu:666).  The CG steps are those of
+ * conj_grad (cg.cu:689-772); the stop test and the restarts are this
+ * library's.  The context's p, q, r are used as workspace; x and z of the
+ * power loop are left alone.
+ *   NPBCG_SOLVE_FP64   plain CG in FP64 with a device-side stop test; if the
+ *                      true residual misses rtol, CG restarts from it.
+ *   NPBCG_SOLVE_MIXED  FP64 iterative refinement around an inner CG whose
+ *                      matrix, vectors and SpMV are FP32 (rho/alpha/beta, dot
+ *                      products and the stop test FP64); the correction is
+ *                      accumulated and the residual recomputed in FP64.
+ *                      Needs the LDS-staged layout (spmv_variant 5). */
+enum { NPBCG_SOLVE_FP64 = 0, NPBCG_SOLVE_MIXED = 1 };
+
+typedef struct npbcg_solve_stats {
+    int converged;     /* 1 if the FP64 true relative residual <= rtol */
+    int iterations;    /* CG steps (inner FP32 steps for MIXED) */
+    int sweeps;        /* refinement sweeps, or restarts + 1 for FP64 */
+    int fp64_spmvs;    /* FP64 matrix products: CG steps and true residuals */
+    int fp32_spmvs;    /* FP32 matrix products (MIXED) */
+    int reserved_i[3];
+    double relres;     /* final ||b - A x|| / ||b|| in FP64 */
+    double time_ms;    /* HIP events around the device work of the solve */
+    double reserved_d[4];
+} npbcg_solve_stats;
+
+/* Uploads the right-hand side b (n entries) and records ||b||. */
+int npbcg_set_rhs(npbcg_ctx *ctx, const double *b);
+/* Solves with the resident b; maxit bounds the CG steps (inner steps for
+ * MIXED).  Returns NPBCG_ERR_NOCONV (stats filled) if rtol was not reached. */
+int npbcg_solve(npbcg_ctx *ctx, int mode, double rtol, int maxit,
+                npbcg_solve_stats *stats);
+/* Downloads the last solution (n entries). */
+int npbcg_get_solution(npbcg_ctx *ctx, double *x);
 
 /* ---- reference-shaped shim ------------------------------------------------
  * For a caller that keeps the reference's exact call

@@ -64,7 +64,7 @@ struct npbcg_ctx {
     double *d_rows_part = nullptr; // nblk * n per-block row sums
     bool cols_sorted = true;
     long long sell_stored = 0;     // SELL entries, padding included
-    double *d_vec = nullptr;   // x, z, p, q, r, p_alt back to back, stride vstride
+    double *d_vec = nullptr;   // x, z, p, q, r, p_alt, b, xsol back to back, stride vstride
     size_t vstride = 0;
     double *d_part = nullptr;  // all partial-sum buffers, carved by refresh_problem
     int part_vec = 0, part_spmv = 0;
@@ -90,7 +90,22 @@ struct npbcg_ctx {
     const double *h_a_ptr = nullptr;
     std::vector<int> colidx_sample;
     std::vector<double> a_sample;
+
+    // solve to a tolerance
+    double rhs_norm = -1.0;        // ||b|| of the resident right-hand side
+    Fp64Problem pbs{};             // solve view: x -> b, z -> solution
+    npbcg_dev::MixedProblem mp{};  // FP32 side of the mixed solve (lazy)
+    float *d_val32 = nullptr;
+    float *d_vec32 = nullptr;      // d, r, p, q (FP32), stride vstride
+    float *d_rows32 = nullptr;
+    double *d_part32 = nullptr;
+    Scalars *d_sc32 = nullptr;
 };
+
+// Device vectors: x, z, p, q, r of the reference plus p_alt (fused p
+// update), b and the solution of npbcg_solve.
+constexpr int kNumVectors = 8;
+constexpr int kVecPAlt = 5, kVecB = 6, kVecXsol = 7;
 
 namespace {
 
@@ -120,7 +135,7 @@ void refresh_problem(npbcg_ctx *c) {
     pb.p = vec_ptr(c, NPBCG_VEC_P);
     pb.q = vec_ptr(c, NPBCG_VEC_Q);
     pb.r = vec_ptr(c, NPBCG_VEC_R);
-    pb.p_alt = vec_ptr(c, NPBCG_VEC_R + 1);
+    pb.p_alt = vec_ptr(c, kVecPAlt);
     double *pp = c->d_part;
     const size_t nv = c->part_vec, ns = c->part_spmv;
     pb.part.xx = pp;
@@ -133,6 +148,15 @@ void refresh_problem(npbcg_ctx *c) {
     pb.zeta_hist = c->d_hist;
     pb.rnorm_hist = c->d_hist ? c->d_hist + c->hist_cap : nullptr;
     pb.hist_cap = c->hist_cap;
+}
+
+// The solve view of the problem: the right-hand side sits where conj_grad
+// keeps x and the iterate where it keeps z.
+void refresh_solve_view(npbcg_ctx *c) {
+    c->pbs = c->pb;
+    c->pbs.x = vec_ptr(c, kVecB);
+    c->pbs.z = vec_ptr(c, kVecXsol);
+    c->pbs.fuse_p_update = 0;  // a solve may stop after any step; p stays in place
 }
 
 void drop_graph(npbcg_ctx *c) {
@@ -202,6 +226,7 @@ int ensure_history(npbcg_ctx *c, int niter) {
     NPBCG_HIP(hipMalloc(&c->d_hist, sizeof(double) * 2 * (size_t)cap));
     c->hist_cap = cap;
     refresh_problem(c);
+    refresh_solve_view(c);
     return NPBCG_OK;
 }
 
@@ -408,7 +433,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     CREATE_HIP(hipMalloc(&c->d_rowstr, sizeof(int) * ((size_t)n + 1)));
     CREATE_HIP(hipMalloc(&c->d_colidx, sizeof(int) * std::max<long long>(1, nnz)));
     CREATE_HIP(hipMalloc(&c->d_a, sizeof(double) * std::max<long long>(1, nnz)));
-    CREATE_HIP(hipMalloc(&c->d_vec, sizeof(double) * 6 * c->vstride));
+    CREATE_HIP(hipMalloc(&c->d_vec, sizeof(double) * kNumVectors * c->vstride));
     c->part_vec = vec_blocks;
     c->part_spmv = variant == npbcg_dev::kSpmvTiled ? std::max(sd.grid, vec_blocks)
                                                     : spmv_blocks;
@@ -455,7 +480,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
         CREATE_HIP(hipMemcpyAsync(c->d_a, a, sizeof(double) * nnz, hipMemcpyHostToDevice,
                                   c->stream));
     }
-    CREATE_HIP(hipMemsetAsync(c->d_vec, 0, sizeof(double) * 6 * c->vstride, c->stream));
+    CREATE_HIP(hipMemsetAsync(c->d_vec, 0, sizeof(double) * kNumVectors * c->vstride, c->stream));
     CREATE_HIP(hipMemsetAsync(c->d_sc, 0, npbcg_dev::scalars_bytes(), c->stream));
 
     refresh_problem(c);
@@ -464,6 +489,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     c->pb.spmv_variant = variant;
     c->pb.fuse_p_update = c->opts.fuse_p_update ? 1 : 0;
     c->pb.sell = sd;
+    refresh_solve_view(c);
     CREATE_HIP(npbcg_dev::launch_fill(c->pb.x, n + 1, 1.0, vec_blocks, c->stream));
     CREATE_HIP(hipStreamSynchronize(c->stream));
 #undef CREATE_HIP
@@ -496,6 +522,11 @@ void npbcg_destroy(npbcg_ctx *c) {
     (void)hipFree(c->d_sell_val);
     (void)hipFree(c->d_sell_col);
     (void)hipFree(c->d_rows_part);
+    (void)hipFree(c->d_val32);
+    (void)hipFree(c->d_vec32);
+    (void)hipFree(c->d_rows32);
+    (void)hipFree(c->d_part32);
+    (void)hipFree(c->d_sc32);
     (void)hipFree(c->d_vec);
     (void)hipFree(c->d_part);
     (void)hipFree(c->d_sc);
@@ -752,6 +783,175 @@ int npbcg_copy_bench(npbcg_ctx *c, long long bytes, int reps, double *avg_ms) {
         return set_error(e == hipErrorOutOfMemory ? NPBCG_ERR_NOMEM : NPBCG_ERR_DEVICE,
                          std::string("npbcg_copy_bench: ") + hipGetErrorString(e));
     *avg_ms = ms / reps;
+    clear_error();
+    return NPBCG_OK;
+}
+
+// ---- solve to a tolerance ---------------------------------------------------
+
+namespace {
+
+int ensure_mixed(npbcg_ctx *c) {
+    if (c->d_val32) return NPBCG_OK;
+    const npbcg_dev::SellDev &sd = c->pb.sell;
+    const size_t stored = (size_t)std::max<long long>(1, c->sell_stored);
+    const size_t vb = (size_t)c->pb.vec_blocks;
+    NPBCG_HIP(hipMalloc(&c->d_val32, sizeof(float) * stored));
+    NPBCG_HIP(hipMalloc(&c->d_vec32, sizeof(float) * 4 * c->vstride));
+    NPBCG_HIP(hipMalloc(&c->d_rows32, sizeof(float) * (size_t)sd.nblk * c->n));
+    NPBCG_HIP(hipMalloc(&c->d_part32, sizeof(double) * (3 * vb + (size_t)sd.grid)));
+    NPBCG_HIP(hipMalloc(&c->d_sc32, npbcg_dev::scalars_bytes()));
+    NPBCG_HIP(hipMemsetAsync(c->d_vec32, 0, sizeof(float) * 4 * c->vstride, c->stream));
+    NPBCG_HIP(hipMemsetAsync(c->d_sc32, 0, npbcg_dev::scalars_bytes(), c->stream));
+    NPBCG_HIP(npbcg_dev::launch_to_fp32(c->d_sell_val, c->d_val32, c->sell_stored, c->stream));
+    // FP32 vector block (padded to 4 entries) + one FP64 slot per slice
+    const int slices_lds = (sd.lds_bytes / 8) - sd.W;  // max slices per workgroup
+    const int lds32 = ((sd.W + 3) & ~3) * 4 + slices_lds * 8;
+    NPBCG_HIP(npbcg_dev::prepare_mixed_kernels(lds32));
+    npbcg_dev::MixedProblem &mp = c->mp;
+    mp.val = c->d_val32;
+    mp.d = c->d_vec32;
+    mp.r = c->d_vec32 + c->vstride;
+    mp.p = c->d_vec32 + 2 * c->vstride;
+    mp.q = c->d_vec32 + 3 * c->vstride;
+    mp.rows_part = c->d_rows32;
+    mp.part_xx = c->d_part32;
+    mp.part_rr[0] = c->d_part32 + vb;
+    mp.part_rr[1] = c->d_part32 + 2 * vb;
+    mp.part_pq = c->d_part32 + 3 * vb;
+    mp.sc = c->d_sc32;
+    mp.lds_bytes = lds32;
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    return NPBCG_OK;
+}
+
+int read_scalars_from(npbcg_ctx *c, const Scalars *dev, Scalars *out) {
+    NPBCG_HIP(hipMemcpyAsync(c->h_stage, dev, sizeof(Scalars), hipMemcpyDeviceToHost,
+                             c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_stage, sizeof(Scalars));
+    return NPBCG_OK;
+}
+
+}  // namespace
+
+int npbcg_set_rhs(npbcg_ctx *c, const double *b) {
+    if (!c || !b) return set_error(NPBCG_ERR_INVALID, "npbcg_set_rhs: NULL argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    double s = 0.0;
+    for (int j = 0; j < c->n; j++) s += b[j] * b[j];
+    c->rhs_norm = std::sqrt(s);
+    const size_t bytes = sizeof(double) * (size_t)c->n;
+    std::memcpy(c->h_stage, b, bytes);
+    NPBCG_HIP(hipMemcpyAsync(c->pbs.x, c->h_stage, bytes, hipMemcpyHostToDevice, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_get_solution(npbcg_ctx *c, double *x) {
+    if (!c || !x) return set_error(NPBCG_ERR_INVALID, "npbcg_get_solution: NULL argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    const size_t bytes = sizeof(double) * (size_t)c->n;
+    NPBCG_HIP(hipMemcpyAsync(c->h_stage, c->pbs.z, bytes, hipMemcpyDeviceToHost, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(x, c->h_stage, bytes);
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats *st) {
+    if (!c || !(rtol > 0.0) || maxit < 1 ||
+        (mode != NPBCG_SOLVE_FP64 && mode != NPBCG_SOLVE_MIXED))
+        return set_error(NPBCG_ERR_INVALID, "npbcg_solve: invalid argument");
+    if (c->rhs_norm < 0.0)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_solve: no right-hand side (npbcg_set_rhs)");
+    if (mode == NPBCG_SOLVE_MIXED && c->pb.spmv_variant != npbcg_dev::kSpmvTiled)
+        return set_error(NPBCG_ERR_INVALID,
+                         "npbcg_solve: mixed precision needs the LDS-staged layout");
+    NPBCG_HIP(hipSetDevice(c->device));
+    npbcg_solve_stats s{};
+    if (c->rhs_norm == 0.0) {  // x = 0 solves b = 0
+        NPBCG_HIP(npbcg_dev::launch_fill(c->pbs.z, c->n + 1, 0.0, c->pb.vec_blocks, c->stream));
+        NPBCG_HIP(hipStreamSynchronize(c->stream));
+        s.converged = 1;
+        if (st) *st = s;
+        clear_error();
+        return NPBCG_OK;
+    }
+    if (mode == NPBCG_SOLVE_MIXED) {
+        int rc = ensure_mixed(c);
+        if (rc != NPBCG_OK) return rc;
+    }
+    const Fp64Problem &pbs = c->pbs;
+    const int batch0 = c->opts.solve_batch > 0 ? c->opts.solve_batch : 10;
+    float ms_total = 0.f;
+    Scalars sc{};
+    double relres = 1.0;
+    if (mode == NPBCG_SOLVE_FP64) {
+        NPBCG_HIP(npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 1));
+        NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
+        NPBCG_HIP(npbcg_dev::launch_cg_init(pbs, 0, c->stream));
+        int batch = std::min(batch0, maxit);
+        bool first = true;
+        for (;;) {
+            NPBCG_HIP(npbcg_dev::enqueue_solve_steps(pbs, batch, first ? 1 : 0, c->stream));
+            NPBCG_HIP(npbcg_dev::launch_resid(pbs, 0, c->stream));
+            NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
+            int rc = read_scalars_from(c, pbs.sc, &sc);
+            if (rc != NPBCG_OK) return rc;
+            s.sweeps++;
+            s.fp64_spmvs = sc.steps + s.sweeps;
+            relres = sc.rnorm / c->rhs_norm;
+            if (relres <= rtol || sc.steps >= maxit) break;
+            // the true residual missed rtol: restart CG from it
+            NPBCG_HIP(npbcg_dev::launch_clear_done(pbs, c->stream));
+            NPBCG_HIP(npbcg_dev::launch_residual_restart(pbs, c->stream));
+            batch = std::min(std::max(batch0 / 2, 2), maxit - sc.steps);
+            first = false;
+        }
+        s.iterations = sc.steps;
+    } else {
+        const npbcg_dev::MixedProblem &mp = c->mp;
+        Fp64Problem inner = pbs;  // same launch shapes, inner scalars
+        inner.sc = mp.sc;
+        NPBCG_HIP(npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 0));
+        NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
+        NPBCG_HIP(npbcg_dev::launch_fill(pbs.z, c->n + 1, 0.0, c->pb.vec_blocks, c->stream));
+        int inner_steps = 0;
+        for (int sweep = 0;; sweep++) {
+            // Inner tolerance relative to this sweep's residual: FP32 holds
+            // about 1e-6 of it; ask for no more than the outer target needs.
+            const double tol_in = std::max(1e-6, 0.5 * rtol / relres);
+            const int steps = std::min(sweep == 0 ? batch0 - 2 : batch0 / 2 + 1,
+                                       maxit - inner_steps);
+            NPBCG_HIP(npbcg_dev::launch_reset_scalars(inner, 0.0, c->stream, tol_in, 1));
+            NPBCG_HIP(npbcg_dev::enqueue_inner32(pbs, mp, sweep == 0 ? 1 : 0, steps,
+                                                 c->stream));
+            NPBCG_HIP(npbcg_dev::launch_refine(pbs, mp, c->stream));
+            NPBCG_HIP(npbcg_dev::launch_resid(pbs, 0, c->stream));
+            NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
+            Scalars si{};
+            int rc = read_scalars_from(c, mp.sc, &si);
+            if (rc != NPBCG_OK) return rc;
+            rc = read_scalars_from(c, pbs.sc, &sc);
+            if (rc != NPBCG_OK) return rc;
+            inner_steps += si.steps;
+            s.sweeps++;
+            relres = sc.rnorm / c->rhs_norm;
+            if (relres <= rtol || inner_steps >= maxit || steps <= 0) break;
+        }
+        s.iterations = inner_steps;
+        s.fp32_spmvs = inner_steps;
+        s.fp64_spmvs = s.sweeps;
+    }
+    NPBCG_HIP(hipEventElapsedTime(&ms_total, c->ev_start, c->ev_stop));
+    s.relres = relres;
+    s.time_ms = ms_total;
+    s.converged = relres <= rtol ? 1 : 0;
+    if (st) *st = s;
+    if (!s.converged)
+        return set_error(NPBCG_ERR_NOCONV, "npbcg_solve: tolerance not reached within maxit");
     clear_error();
     return NPBCG_OK;
 }

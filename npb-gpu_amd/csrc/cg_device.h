@@ -59,8 +59,13 @@ struct Scalars {
     double xscale;   // 1/sqrt(z.z)  (norm_temp2)
     double zeta;     // shift + 1/(x.z)
     double shift;
+    double tol2;     // solve: stop when r.r <= tol2 (= rtol^2 * r0.r0)
+    double rtol2;    // solve: rtol^2, set before the solve
+    double bb;       // solve: r0.r0 of the step that set tol2 (b.b for x0 = 0)
     int iter;        // outer iterations recorded so far in the history
-    int pad;
+    int stop_enabled;  // solve mode: CG steps stop once r.r <= tol2
+    int done;        // set when the stop test passed; later CG steps return at once
+    int steps;       // CG steps executed since the last reset
 };
 
 // Sum of this workgroup's per-thread values, written to part[blockIdx.x]

@@ -63,7 +63,8 @@ __global__ __launch_bounds__(kBlock) void spmv_csr_kernel(
     const int *__restrict__ rowstr, const int *__restrict__ colidx,
     const double *__restrict__ a, const double *__restrict__ v,
     double *__restrict__ y, const double *__restrict__ x, int n,
-    double *__restrict__ part) {
+    double *__restrict__ part, const Scalars *__restrict__ sc) {
+    if (MODE == 0 && sc->done) return;  // a solve already met its tolerance
     constexpr int kGroups = kBlock / SEG;
     const int sl = threadIdx.x % SEG;
     // With a whole wavefront per row the row index is wave-uniform; saying so
@@ -125,7 +126,9 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     const uint16_t *__restrict__ col, const double *__restrict__ v,
     double *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W,
     int nrange, const double *__restrict__ r, double *__restrict__ p_new,
-    const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec) {
+    const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
+    const Scalars *__restrict__ sc) {
+    if (MODE == 0 && sc->done) return;  // a solve already met its tolerance
     // Dynamic LDS: W vector entries, then one p.q slot per slice of this
     // workgroup (the p.q partial is summed in slice order).
     extern __shared__ double2 lds_v2[];  // 16-byte aligned
@@ -261,7 +264,8 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     const double *__restrict__ p, double *__restrict__ q, double *__restrict__ z,
     double *__restrict__ r, int n, const double *__restrict__ rho_part, int n_rho,
     const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
-    const double *__restrict__ rows_part, int nblk, Scalars *sc) {
+    const double *__restrict__ rows_part, int nblk, Scalars *sc, int first_step) {
+    if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rho_part, n_rho);
     const double d = reduce_partials<kBlock>(pq_part, n_pq);
     const double alpha = rho / d;
@@ -269,6 +273,11 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
         sc->rho0 = rho;
         sc->d = d;
         sc->alpha = alpha;
+        if (first_step) {
+            sc->tol2 = sc->rtol2 * rho;
+            sc->bb = rho;
+        }  // rho = r0.r0 of this start
+        sc->steps += 1;
     }
     double acc = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
@@ -294,13 +303,19 @@ __global__ __launch_bounds__(kBlock) void update_p_kernel(
     const double *__restrict__ r, const double *p_in, double *p_out, int n,
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
     Scalars *sc) {
+    if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rr_part, n_vec);
     const double rho0 = reduce_partials<kBlock>(rho_part, n_vec);
     const double beta = rho / rho0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->rho = rho;
         sc->beta = beta;
+        // Solve mode: the recursive residual met the tolerance; the remaining
+        // steps of the launched sequence return at entry.  p is left as is
+        // (the solution z is final).
+        if (sc->stop_enabled && rho <= sc->tol2) sc->done = 1;
     }
+    if (sc->stop_enabled && rho <= sc->tol2) return;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
         p_out[j] = r[j] + beta * p_in[j];
 }
@@ -403,11 +418,21 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const double2 *__restrict_
 
 // Resets the outer-loop bookkeeping.  A kernel rather than a memset so it may
 // sit in a captured graph.
-__global__ void reset_scalars_kernel(Scalars *sc, double shift) {
+__global__ void reset_scalars_kernel(Scalars *sc, double shift, double rtol2, int stop) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         sc->iter = 0;
         sc->shift = shift;
+        sc->rtol2 = rtol2;
+        sc->tol2 = 0.0;
+        sc->stop_enabled = stop;
+        sc->done = 0;
+        sc->steps = 0;
     }
+}
+
+// Clears the stop flag (a solve restarting from its true residual).
+__global__ void clear_done_kernel(Scalars *sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) sc->done = 0;
 }
 
 __global__ void fill_kernel(double *__restrict__ v, int count, double value) {
@@ -440,7 +465,8 @@ hipError_t launch_csr_t(const Fp64Problem &pb, hipStream_t st, hipEvent_t e0, hi
     double *y = MODE == 0 ? pb.q : pb.r;
     double *part = MODE == 0 ? pb.part.pq : pb.part.res;
     return launch_k(spmv_csr_kernel<SEG, UNR, MODE>, pb.spmv_blocks, kBlock, 0, st, e0, e1,
-                    pb.rowstr, pb.colidx, pb.a, v, y, pb.x, pb.n, part);
+                    pb.rowstr, pb.colidx, pb.a, v, y, pb.x, pb.n, part,
+                    (const Scalars *)pb.sc);
 }
 
 bool fused_p(const Fp64Problem &pb) {
@@ -471,13 +497,14 @@ hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEve
                         (const double *)p_of_step(pb, step - 1), d.rows_part, pb.part.pq,
                         pb.n, d.W, d.nrange, (const double *)pb.r, p_of_step(pb, step),
                         (const double *)pb.part.rr[(step - 1) & 1],
-                        rho_source(pb, step - 1), pb.vec_blocks);
+                        rho_source(pb, step - 1), pb.vec_blocks, (const Scalars *)pb.sc);
     }
     const double *v = MODE == 0 ? p_of_step(pb, step) : pb.z;
     return launch_k(spmv_sell_kernel<BLOCK, 8, MODE, false>, d.grid, BLOCK, d.lds_bytes, st,
                     e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v, d.rows_part,
                     pb.part.pq, pb.n, d.W, d.nrange, (const double *)nullptr,
-                    (double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0);
+                    (double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0,
+                    (const Scalars *)pb.sc);
 }
 
 template <int MODE>
@@ -565,12 +592,14 @@ hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st) {
+hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
+                            int sets_tol) {
     update_zr_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
         p_of_step(pb, step), pb.q, pb.z, pb.r, pb.n, rho_source(pb, step), pb.vec_blocks,
         pb.part.pq,
         spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part,
-        pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc);
+        pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc,
+        (step == 0 && sets_tol) ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -597,8 +626,14 @@ hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_scalars,
     return hipGetLastError();
 }
 
-hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st) {
-    reset_scalars_kernel<<<1, 64, 0, st>>>(pb.sc, shift);
+hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st,
+                                double rtol, int stop) {
+    reset_scalars_kernel<<<1, 64, 0, st>>>(pb.sc, shift, rtol * rtol, stop);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_done(const Fp64Problem &pb, hipStream_t st) {
+    clear_done_kernel<<<1, 64, 0, st>>>(pb.sc);
     return hipGetLastError();
 }
 
@@ -628,6 +663,17 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
         if (e == hipSuccess && (!fused || it == cgitmax - 1)) e = launch_update_p(pb, it, st);
     }
     if (e == hipSuccess) e = launch_resid(pb, record, st);
+    return e;
+}
+
+hipError_t enqueue_solve_steps(const Fp64Problem &pb, int steps, int sets_tol,
+                               hipStream_t st) {
+    hipError_t e = hipSuccess;
+    for (int it = 0; it < steps && e == hipSuccess; it++) {
+        e = launch_spmv<0>(pb, st, nullptr, nullptr, it);
+        if (e == hipSuccess) e = launch_update_zr(pb, it, st, sets_tol);
+        if (e == hipSuccess) e = launch_update_p(pb, it, st);
+    }
     return e;
 }
 

@@ -89,13 +89,20 @@ hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st);
 // forms rnorm, zeta, xscale and, with record != 0, appends to the history.
 hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st);
 hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st);
-hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st);
+// CG step `step` of a sequence; with sets_tol, step 0 also fixes the solve's
+// stop level from its r0.r0.
+hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
+                            int sets_tol = 1);
 hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st);
 hipError_t launch_norms(const Fp64Problem &pb, hipStream_t st);
 // x = s * z, or x = xscale * z (the last residual finish) if from_scalars.
 hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_scalars,
                           hipStream_t st);
-hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st);
+// Resets the outer-loop and solve bookkeeping; stop != 0 enables the
+// relative-residual stop test r.r <= rtol^2 * r0.r0 inside the CG steps.
+hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st,
+                                double rtol = 0.0, int stop = 0);
+hipError_t launch_clear_done(const Fp64Problem &pb, hipStream_t st);
 hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream_t st);
 // Copy of bytes (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
@@ -105,6 +112,33 @@ hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
 // (spmv, update_zr, update_p), residual SpMV and finish.
 hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, int record,
                              hipStream_t st, const SpmvProbe *probe);
+
+// `steps` CG steps (SpMV, update_zr, update_p) of a solve on pb (x holds b,
+// z the iterate); the stop flag makes steps after convergence return at once.
+hipError_t enqueue_solve_steps(const Fp64Problem &pb, int steps, int sets_tol,
+                               hipStream_t st);
+
+// FP32 side of the mixed-precision solve (cg_mixed.hip).
+struct MixedProblem {
+    const float *val = nullptr;   // FP32 values in the SELL order of the FP64 copy
+    float *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;  // n+2 each
+    float *rows_part = nullptr;   // nblk * n per-block row sums
+    double *part_xx = nullptr;    // r0.r0 partials of an inner start [vec_blocks]
+    double *part_rr[2] = {nullptr, nullptr};  // r.r partials [vec_blocks]
+    double *part_pq = nullptr;    // p.q partials [sell grid]
+    Scalars *sc = nullptr;        // inner-solve scalars
+    int lds_bytes = 0;
+};
+hipError_t prepare_mixed_kernels(int lds_bytes);
+hipError_t launch_to_fp32(const double *src, float *dst, long long count, hipStream_t st);
+// Inner FP32 CG on A d = b - A x (or b on the first sweep): init and `steps`
+// steps; pb is the solve view (x holds b, z the FP64 iterate, r = A z).
+hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int first_sweep,
+                           int steps, hipStream_t st);
+// z += d (the FP64 iterate takes the FP32 correction).
+hipError_t launch_refine(const Fp64Problem &pb, const MixedProblem &mp, hipStream_t st);
+// r = b - r (r held A z), p = r, r.r partials into part.xx.
+hipError_t launch_residual_restart(const Fp64Problem &pb, hipStream_t st);
 
 size_t scalars_bytes();
 

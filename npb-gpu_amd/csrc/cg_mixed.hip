@@ -1,0 +1,251 @@
+// Mixed-precision CG kernels (FP32 operator and vectors, FP64 scalars).
+//
+// The mixed solve is FP64 iterative refinement around an inner CG that runs
+// on an FP32 copy of the matrix:
+//   x = 0, r = b                                       (FP64)
+//   repeat: inner CG on A d = r in FP32 (matrix, vectors, SpMV accumulation),
+//           with rho, alpha, beta, the dot products and the stop test in FP64;
+//           x += d and r = b - A x with the FP64 matrix   (true residual)
+//   until ||r|| <= rtol ||b||.
+// The FP32 matrix shares the FP64 SELL layout's slices and 16-bit columns;
+// only the values are FP32 (6 bytes per stored entry instead of 10).  Kernel
+// structure and the partial-sum handoff are those of the FP64 path.
+#include "cg_device.h"
+#include "cg_kernels.h"
+
+namespace npbcg_dev {
+
+// val32 = (float) val64 (the FP32 operator).
+__global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double *__restrict__ src,
+                                                         float *__restrict__ dst,
+                                                         long long count) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < count;
+         i += (long long)gridDim.x * kBlock)
+        dst[i] = (float)src[i];
+}
+
+// q = A p in FP32 over the SELL layout (see spmv_sell_kernel): per-block row
+// sums into rows32[b][row], FP64 partials of p.q.
+template <int BLOCK, int UNR>
+__global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
+    const int *__restrict__ wg_slice, const int *__restrict__ slice_off,
+    const int *__restrict__ slice_row, const float *__restrict__ val,
+    const uint16_t *__restrict__ col, const float *__restrict__ v,
+    float *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W, int nrange,
+    const Scalars *__restrict__ sc) {
+    if (sc->done) return;
+    extern __shared__ double2 lds_v2[];  // 16-byte aligned
+    float *vblk = reinterpret_cast<float *>(lds_v2);
+    double *slice_dot = reinterpret_cast<double *>(vblk + ((W + 3) & ~3));
+    constexpr int kWaves = BLOCK / kWave;
+    const int wg = blockIdx.x;
+    const int b = wg / nrange;
+    const int c0 = b * W;
+    const int wb = min(W, n - c0);
+    {
+        // c0 is even (W even); stage pairs, then a possible odd tail.
+        const float2 *v2 = reinterpret_cast<const float2 *>(v + c0);
+        float2 *l2 = reinterpret_cast<float2 *>(vblk);
+#pragma unroll 4
+        for (int i = threadIdx.x; i < wb / 2; i += BLOCK) l2[i] = v2[i];
+        if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    float *part_b = rows_part + (size_t)b * n;
+    const int s0 = wg_slice[wg];
+    const int nsl = wg_slice[wg + 1] - s0;
+    const int rounds = (nsl + kWaves - 1) / kWaves;
+    for (int rd = 0; rd < rounds; rd++) {
+        const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
+        if (k >= nsl) continue;
+        const int s = s0 + k;
+        const int base = slice_off[s];
+        const int width = (slice_off[s + 1] - base) / kWave;
+        const float *vp = val + base + lane;
+        const uint16_t *cp = col + base + lane;
+        float acc = 0.0f;
+        int j = 0;
+        for (; j + UNR <= width; j += UNR) {
+            float av[UNR];
+            int c[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                av[u] = vp[(size_t)(j + u) * kWave];
+                c[u] = cp[(size_t)(j + u) * kWave];
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; u++) acc = fmaf(av[u], vblk[c[u]], acc);
+        }
+        for (; j < width; j++) acc = fmaf(vp[(size_t)j * kWave], vblk[cp[(size_t)j * kWave]], acc);
+        const int row = slice_row[(size_t)s * kWave + lane];
+        if (row >= 0) part_b[row] = acc;
+        const double d = segment_sum<64>(row >= 0 ? (double)v[row] * (double)acc : 0.0);
+        if (lane == 0) slice_dot[k] = d;
+    }
+    __syncthreads();
+    double dot = 0.0;
+    for (int i = threadIdx.x; i < nsl; i += BLOCK) dot += slice_dot[i];
+    store_block_partial<BLOCK>(dot, dot_part);
+}
+
+// Start of an inner solve: d = 0, r = p = fp32(b - Ax) (or fp32(b) on the
+// first sweep, x = 0), FP64 partials of r.r.
+__global__ __launch_bounds__(kBlock) void init32_kernel(
+    const double *__restrict__ b, const double *__restrict__ ax, int first, int n,
+    float *__restrict__ d, float *__restrict__ r, float *__restrict__ p,
+    double *__restrict__ rr_out) {
+    double acc = 0.0;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const float rj = (float)(first ? b[j] : b[j] - ax[j]);
+        d[j] = 0.0f;
+        r[j] = rj;
+        p[j] = rj;
+        acc += (double)rj * (double)rj;
+    }
+    store_block_partial<kBlock>(acc, rr_out);
+}
+
+// alpha = rho / (p.q) from FP64 partials; q = sum of the block row sums;
+// d += alpha p; r -= alpha q; FP64 partials of the new r.r.  The first step
+// of an inner solve sets its stop level tol2 = rtol2 * r0.r0.
+__global__ __launch_bounds__(kBlock) void update_zr32_kernel(
+    const float *__restrict__ p, float *__restrict__ q, float *__restrict__ d,
+    float *__restrict__ r, int n, const double *__restrict__ rho_part,
+    const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
+    const float *__restrict__ rows_part, int nblk, Scalars *sc, int first_step) {
+    if (sc->done) return;
+    const double rho = reduce_partials<kBlock>(rho_part, gridDim.x);
+    const double dd = reduce_partials<kBlock>(pq_part, n_pq);
+    const double alpha = rho / dd;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->rho0 = rho;
+        sc->d = dd;
+        sc->alpha = alpha;
+        if (first_step) {
+            sc->tol2 = sc->rtol2 * rho;
+            sc->bb = rho;
+        }
+        sc->steps += 1;
+    }
+    const float a32 = (float)alpha;
+    double acc = 0.0;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        float qj = rows_part[j];
+        for (int b = 1; b < nblk; b++) qj += rows_part[(size_t)b * n + j];
+        q[j] = qj;
+        d[j] = fmaf(a32, p[j], d[j]);
+        const float rj = fmaf(-a32, qj, r[j]);
+        r[j] = rj;
+        acc += (double)rj * (double)rj;
+    }
+    store_block_partial<kBlock>(acc, rr_out);
+}
+
+// beta = (new r.r) / (old r.r) in FP64; p = r + beta p; stop test.
+__global__ __launch_bounds__(kBlock) void update_p32_kernel(
+    const float *__restrict__ r, float *__restrict__ p, int n,
+    const double *__restrict__ rr_part, const double *__restrict__ rho_part, Scalars *sc) {
+    if (sc->done) return;
+    const double rho = reduce_partials<kBlock>(rr_part, gridDim.x);
+    const double rho0 = reduce_partials<kBlock>(rho_part, gridDim.x);
+    const double beta = rho / rho0;
+    const bool stop = rho <= sc->tol2;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->rho = rho;
+        sc->beta = beta;
+        if (stop) sc->done = 1;
+    }
+    if (stop) return;
+    const float b32 = (float)beta;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        p[j] = fmaf(b32, p[j], r[j]);
+}
+
+// x += d (FP64 accumulation of the FP32 correction).
+__global__ __launch_bounds__(kBlock) void refine_kernel(double *__restrict__ x,
+                                                       const float *__restrict__ d, int n) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        x[j] += (double)d[j];
+}
+
+// r = b - A z (in place over r = A z), p = r, partials of r.r: restart of an
+// FP64 solve from its true residual.
+__global__ __launch_bounds__(kBlock) void residual_restart_kernel(
+    const double *__restrict__ b, double *__restrict__ r, double *__restrict__ p, int n,
+    double *__restrict__ rr_out) {
+    double acc = 0.0;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const double rj = b[j] - r[j];
+        r[j] = rj;
+        p[j] = rj;
+        acc += rj * rj;
+    }
+    store_block_partial<kBlock>(acc, rr_out);
+}
+
+// ---------------------------------------------------------------------------
+
+hipError_t prepare_mixed_kernels(int lds_bytes) {
+    hipError_t e1 = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, 8>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    hipError_t e2 = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&spmv_sell32_kernel<512, 8>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    return e1 != hipSuccess ? e1 : e2;
+}
+
+hipError_t launch_to_fp32(const double *src, float *dst, long long count, hipStream_t st) {
+    to_fp32_kernel<<<2048, kBlock, 0, st>>>(src, dst, count);
+    return hipGetLastError();
+}
+
+static const double *rho32_source(const MixedProblem &mp, int step) {
+    return step == 0 ? mp.part_xx : mp.part_rr[(step - 1) & 1];
+}
+
+hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int first_sweep,
+                           int steps, hipStream_t st) {
+    const int vb = pb.vec_blocks;
+    init32_kernel<<<vb, kBlock, 0, st>>>(pb.x, pb.r, first_sweep, pb.n, mp.d, mp.r, mp.p,
+                                         mp.part_xx);
+    hipError_t e = hipGetLastError();
+    for (int it = 0; it < steps && e == hipSuccess; it++) {
+        const SellDev &sd = pb.sell;
+        if (sd.block == 512)
+            spmv_sell32_kernel<512, 8><<<sd.grid, 512, mp.lds_bytes, st>>>(
+                sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, mp.p, mp.rows_part,
+                mp.part_pq, pb.n, sd.W, sd.nrange, mp.sc);
+        else
+            spmv_sell32_kernel<1024, 8><<<sd.grid, 1024, mp.lds_bytes, st>>>(
+                sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, mp.p, mp.rows_part,
+                mp.part_pq, pb.n, sd.W, sd.nrange, mp.sc);
+        e = hipGetLastError();
+        if (e != hipSuccess) break;
+        update_zr32_kernel<<<vb, kBlock, 0, st>>>(mp.p, mp.q, mp.d, mp.r, pb.n,
+                                                  rho32_source(mp, it), mp.part_pq, sd.grid,
+                                                  mp.part_rr[it & 1], mp.rows_part, sd.nblk,
+                                                  mp.sc, it == 0 ? 1 : 0);
+        e = hipGetLastError();
+        if (e != hipSuccess) break;
+        update_p32_kernel<<<vb, kBlock, 0, st>>>(mp.r, mp.p, pb.n, mp.part_rr[it & 1],
+                                                 rho32_source(mp, it), mp.sc);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t launch_refine(const Fp64Problem &pb, const MixedProblem &mp, hipStream_t st) {
+    refine_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.z, mp.d, pb.n);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_restart(const Fp64Problem &pb, hipStream_t st) {
+    residual_restart_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.r, pb.p, pb.n,
+                                                              pb.part.xx);
+    return hipGetLastError();
+}
+
+}  // namespace npbcg_dev

@@ -47,7 +47,23 @@ class Opts(ctypes.Structure):
                 ("engine", ctypes.c_int), ("spmv_variant", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("tile_rows", ctypes.c_int),
                 ("tile_width", ctypes.c_int), ("fuse_p_update", ctypes.c_int),
-                ("reserved", ctypes.c_int * 8)]
+                ("solve_batch", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+
+
+SOLVE_FP64, SOLVE_MIXED = 0, 1
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("sweeps", ctypes.c_int), ("fp64_spmvs", ctypes.c_int),
+                ("fp32_spmvs", ctypes.c_int), ("reserved_i", ctypes.c_int * 3),
+                ("relres", ctypes.c_double), ("time_ms", ctypes.c_double),
+                ("reserved_d", ctypes.c_double * 4)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k in ("converged", "iterations", "sweeps",
+                                               "fp64_spmvs", "fp32_spmvs", "relres",
+                                               "time_ms")}
 
 
 class Timing(ctypes.Structure):
@@ -87,6 +103,10 @@ def _load() -> ctypes.CDLL:
         "npbcg_spmv_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
         "npbcg_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
         "npbcg_copy_bench": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int, _DP]),
+        "npbcg_set_rhs": (ctypes.c_int, [vp, _F64P]),
+        "npbcg_solve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(SolveStats)]),
+        "npbcg_get_solution": (ctypes.c_int, [vp, _F64P]),
         "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
         "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
                                           _F64P, _F64P, _DP]),
@@ -248,6 +268,24 @@ class Context:
         buf = ctypes.create_string_buffer(1024)
         _check(LIB.npbcg_describe(self._h, buf, 1024), "npbcg_describe")
         return json.loads(buf.value.decode())
+
+    def set_rhs(self, b: np.ndarray):
+        _check(LIB.npbcg_set_rhs(self._h, np.ascontiguousarray(b[: self.n], dtype=np.float64)),
+               "npbcg_set_rhs")
+
+    def solve(self, mode: int = SOLVE_FP64, rtol: float = 1e-10, maxit: int = 1000,
+              check: bool = True):
+        """Solves A x = b (resident b); returns SolveStats."""
+        st = SolveStats()
+        rc = LIB.npbcg_solve(self._h, mode, rtol, maxit, ctypes.byref(st))
+        if check:
+            _check(rc, "npbcg_solve")
+        return st
+
+    def solution(self) -> np.ndarray:
+        x = np.zeros(self.n)
+        _check(LIB.npbcg_get_solution(self._h, x), "npbcg_get_solution")
+        return x
 
     def copy_bench(self, nbytes: int, reps: int = 10) -> float:
         """Mean ms of a device copy of nbytes (read nbytes + write nbytes)."""
