@@ -184,13 +184,25 @@ int npbcg_copy_bench(npbcg_ctx *ctx, long long bytes, int reps, double *avg_ms);
  *                      Needs the LDS-staged layout (spmv_variant 5). */
 enum { NPBCG_SOLVE_FP64 = 0, NPBCG_SOLVE_MIXED = 1 };
 
+/* Diagonal preconditioners M^-1 for npbcg_solve (PCG: rho = r.M^-1 r,
+ * p = M^-1 r + beta p; the stop test uses |r.M^-1 r| <= rtol^2 |r0.M^-1 r0|,
+ * the final check is always the FP64 true residual). */
+enum {
+    NPBCG_PRECOND_NONE = 0,     /* plain CG */
+    NPBCG_PRECOND_JACOBI = 1,   /* M^-1 = 1 / diag(A) */
+    NPBCG_PRECOND_LEARNED = 2   /* M^-1 = exp(g) / diag(A), g from npbcg_net_apply;
+                                   falls back to Jacobi if the solve fails */
+};
+
 typedef struct npbcg_solve_stats {
     int converged;     /* 1 if the FP64 true relative residual <= rtol */
     int iterations;    /* CG steps (inner FP32 steps for MIXED) */
     int sweeps;        /* refinement sweeps, or restarts + 1 for FP64 */
     int fp64_spmvs;    /* FP64 matrix products: CG steps and true residuals */
     int fp32_spmvs;    /* FP32 matrix products (MIXED) */
-    int reserved_i[3];
+    int precond;       /* preconditioner used (NPBCG_PRECOND_*) */
+    int fallback;      /* 1 if a learned-preconditioner solve was redone with Jacobi */
+    int reserved_i;
     double relres;     /* final ||b - A x|| / ||b|| in FP64 */
     double time_ms;    /* HIP events around the device work of the solve */
     double reserved_d[4];
@@ -204,6 +216,11 @@ int npbcg_solve(npbcg_ctx *ctx, int mode, double rtol, int maxit,
                 npbcg_solve_stats *stats);
 /* Downloads the last solution (n entries). */
 int npbcg_get_solution(npbcg_ctx *ctx, double *x);
+/* Selects the preconditioner of later solves.  LEARNED needs a diagonal from
+ * npbcg_net_apply (NPBCG_ERR_INVALID otherwise). */
+int npbcg_set_preconditioner(npbcg_ctx *ctx, int kind);
+/* Downloads the active M^-1 diagonal (n entries; plain CG: ones). */
+int npbcg_get_preconditioner(npbcg_ctx *ctx, double *minv);
 
 /* ---- reference-shaped shim ------------------------------------------------
  * For a caller that keeps the reference's exact call

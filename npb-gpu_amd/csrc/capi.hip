@@ -97,6 +97,10 @@ struct npbcg_ctx {
     npbcg_dev::MixedProblem mp{};  // FP32 side of the mixed solve (lazy)
     float *d_val32 = nullptr;
     float *d_vec32 = nullptr;      // d, r, p, q (FP32), stride vstride
+    int precond = NPBCG_PRECOND_NONE;
+    bool learned_ready = false;
+    double *d_minv = nullptr;      // [0, n) Jacobi, [vstride, ...) learned (FP64)
+    float *d_minv32 = nullptr;     // FP32 copy of the active diagonal
     float *d_rows32 = nullptr;
     double *d_part32 = nullptr;
     Scalars *d_sc32 = nullptr;
@@ -523,6 +527,8 @@ void npbcg_destroy(npbcg_ctx *c) {
     (void)hipFree(c->d_sell_col);
     (void)hipFree(c->d_rows_part);
     (void)hipFree(c->d_val32);
+    (void)hipFree(c->d_minv);
+    (void)hipFree(c->d_minv32);
     (void)hipFree(c->d_vec32);
     (void)hipFree(c->d_rows32);
     (void)hipFree(c->d_part32);
@@ -860,7 +866,111 @@ int npbcg_get_solution(npbcg_ctx *c, double *x) {
     return NPBCG_OK;
 }
 
+namespace {
+
+int ensure_precond_buffers(npbcg_ctx *c) {
+    if (c->d_minv) return NPBCG_OK;
+    NPBCG_HIP(hipMalloc(&c->d_minv, sizeof(double) * 2 * c->vstride));
+    NPBCG_HIP(hipMalloc(&c->d_minv32, sizeof(float) * c->vstride));
+    NPBCG_HIP(npbcg_dev::launch_jacobi(c->pb, c->d_minv, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    return NPBCG_OK;
+}
+
+// Points the solve views at the diagonal of `kind` (FP64 and its FP32 copy).
+int select_precond(npbcg_ctx *c, int kind) {
+    if (kind == NPBCG_PRECOND_NONE) {
+        c->pbs.minv = nullptr;
+        c->mp.minv = nullptr;
+        return NPBCG_OK;
+    }
+    int rc = ensure_precond_buffers(c);
+    if (rc != NPBCG_OK) return rc;
+    const double *m = kind == NPBCG_PRECOND_LEARNED ? c->d_minv + c->vstride : c->d_minv;
+    NPBCG_HIP(npbcg_dev::launch_narrow(m, c->d_minv32, c->n, c->pb.vec_blocks, c->stream));
+    c->pbs.minv = m;
+    c->mp.minv = c->d_minv32;
+    return NPBCG_OK;
+}
+
+int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats *st);
+
+}  // namespace
+
+}  // extern "C"
+
+namespace npbcg_detail {
+
+double *ctx_learned_diag(npbcg_ctx *c) {
+    if (ensure_precond_buffers(c) != NPBCG_OK) return nullptr;
+    c->learned_ready = true;
+    return c->d_minv + c->vstride;
+}
+
+const npbcg_dev::Fp64Problem *ctx_problem(const npbcg_ctx *c) { return &c->pb; }
+const npbcg_dev::Fp64Problem *ctx_solve_view(const npbcg_ctx *c) { return &c->pbs; }
+hipStream_t ctx_stream(const npbcg_ctx *c) { return c->stream; }
+int ctx_device(const npbcg_ctx *c) { return c->device; }
+
+}  // namespace npbcg_detail
+
+extern "C" {
+
+int npbcg_set_preconditioner(npbcg_ctx *c, int kind) {
+    if (!c || kind < NPBCG_PRECOND_NONE || kind > NPBCG_PRECOND_LEARNED)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_set_preconditioner: invalid argument");
+    if (kind == NPBCG_PRECOND_LEARNED && !c->learned_ready)
+        return set_error(NPBCG_ERR_INVALID,
+                         "npbcg_set_preconditioner: no learned diagonal (npbcg_net_apply)");
+    NPBCG_HIP(hipSetDevice(c->device));
+    c->precond = kind;
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_get_preconditioner(npbcg_ctx *c, double *minv) {
+    if (!c || !minv) return set_error(NPBCG_ERR_INVALID, "npbcg_get_preconditioner: NULL");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = select_precond(c, c->precond);
+    if (rc != NPBCG_OK) return rc;
+    if (!c->pbs.minv) {
+        for (int j = 0; j < c->n; j++) minv[j] = 1.0;
+    } else {
+        NPBCG_HIP(hipMemcpyAsync(c->h_stage, c->pbs.minv, sizeof(double) * c->n,
+                                 hipMemcpyDeviceToHost, c->stream));
+        NPBCG_HIP(hipStreamSynchronize(c->stream));
+        std::memcpy(minv, c->h_stage, sizeof(double) * c->n);
+    }
+    clear_error();
+    return NPBCG_OK;
+}
+
 int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats *st) {
+    if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_solve: NULL context");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = select_precond(c, c->precond);
+    if (rc != NPBCG_OK) return rc;
+    npbcg_solve_stats s{};
+    rc = solve_once(c, mode, rtol, maxit, &s);
+    s.precond = c->precond;
+    if (c->precond == NPBCG_PRECOND_LEARNED &&
+        (rc == NPBCG_ERR_NOCONV || !(s.relres == s.relres))) {
+        // The learned diagonal did not get there: Jacobi-CG redoes the solve.
+        const float first_ms = (float)s.time_ms;
+        rc = select_precond(c, NPBCG_PRECOND_JACOBI);
+        if (rc != NPBCG_OK) return rc;
+        rc = solve_once(c, mode, rtol, maxit, &s);
+        s.precond = NPBCG_PRECOND_JACOBI;
+        s.fallback = 1;
+        s.time_ms += first_ms;
+    }
+    if (st) *st = s;
+    return rc;
+}
+
+namespace {
+
+int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats *st) {
     if (!c || !(rtol > 0.0) || maxit < 1 ||
         (mode != NPBCG_SOLVE_FP64 && mode != NPBCG_SOLVE_MIXED))
         return set_error(NPBCG_ERR_INVALID, "npbcg_solve: invalid argument");
@@ -955,6 +1065,8 @@ int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stat
     clear_error();
     return NPBCG_OK;
 }
+
+}  // namespace
 
 // ---- reference-shaped shim -------------------------------------------------
 

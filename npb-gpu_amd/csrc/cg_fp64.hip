@@ -264,7 +264,8 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     const double *__restrict__ p, double *__restrict__ q, double *__restrict__ z,
     double *__restrict__ r, int n, const double *__restrict__ rho_part, int n_rho,
     const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
-    const double *__restrict__ rows_part, int nblk, Scalars *sc, int first_step) {
+    const double *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
+    const double *__restrict__ minv) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rho_part, n_rho);
     const double d = reduce_partials<kBlock>(pq_part, n_pq);
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
         sc->d = d;
         sc->alpha = alpha;
         if (first_step) {
-            sc->tol2 = sc->rtol2 * rho;
+            sc->tol2 = sc->rtol2 * fabs(rho);
             sc->bb = rho;
         }  // rho = r0.r0 of this start
         sc->steps += 1;
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
         z[j] = z[j] + alpha * p[j];
         const double rj = r[j] - alpha * qj;
         r[j] = rj;
-        acc += rj * rj;
+        acc += minv ? rj * (minv[j] * rj) : rj * rj;
     }
     store_block_partial<kBlock>(acc, rr_out);
 }
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
 __global__ __launch_bounds__(kBlock) void update_p_kernel(
     const double *__restrict__ r, const double *p_in, double *p_out, int n,
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
-    Scalars *sc) {
+    Scalars *sc, const double *__restrict__ minv) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rr_part, n_vec);
     const double rho0 = reduce_partials<kBlock>(rho_part, n_vec);
@@ -313,11 +314,16 @@ __global__ __launch_bounds__(kBlock) void update_p_kernel(
         // Solve mode: the recursive residual met the tolerance; the remaining
         // steps of the launched sequence return at entry.  p is left as is
         // (the solution z is final).
-        if (sc->stop_enabled && rho <= sc->tol2) sc->done = 1;
+        if (sc->stop_enabled && fabs(rho) <= sc->tol2) sc->done = 1;
     }
-    if (sc->stop_enabled && rho <= sc->tol2) return;
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
-        p_out[j] = r[j] + beta * p_in[j];
+    if (sc->stop_enabled && fabs(rho) <= sc->tol2) return;
+    if (minv) {
+        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+            p_out[j] = minv[j] * r[j] + beta * p_in[j];
+    } else {
+        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+            p_out[j] = r[j] + beta * p_in[j];
+    }
 }
 
 // Start of conj_grad (cg.cu:671-686): q = z = 0, r = p = x for j <= n and
@@ -326,7 +332,8 @@ __global__ __launch_bounds__(kBlock) void update_p_kernel(
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(
     double *__restrict__ x, double *__restrict__ z, double *__restrict__ p,
     double *__restrict__ q, double *__restrict__ r, int n, int scale_x,
-    const Scalars *__restrict__ sc, double *__restrict__ xx_out) {
+    const Scalars *__restrict__ sc, double *__restrict__ xx_out,
+    const double *__restrict__ minv) {
     const double xs = scale_x ? sc->xscale : 1.0;
     double acc = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j <= n; j += gridDim.x * kBlock) {
@@ -338,8 +345,9 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(
         q[j] = 0.0;
         z[j] = 0.0;
         r[j] = xv;
-        p[j] = xv;
-        if (j < n) acc += xv * xv;
+        const double sv = (minv && j < n) ? minv[j] * xv : xv;  // s = M^-1 r
+        p[j] = sv;
+        if (j < n) acc += xv * sv;
     }
     store_block_partial<kBlock>(acc, xx_out);
 }
@@ -588,7 +596,7 @@ hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
 
 hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st) {
     cg_init_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.p, pb.q, pb.r, pb.n,
-                                                     scale_x, pb.sc, pb.part.xx);
+                                                     scale_x, pb.sc, pb.part.xx, pb.minv);
     return hipGetLastError();
 }
 
@@ -599,7 +607,7 @@ hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
         pb.part.pq,
         spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part,
         pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc,
-        (step == 0 && sets_tol) ? 1 : 0);
+        (step == 0 && sets_tol) ? 1 : 0, pb.minv);
     return hipGetLastError();
 }
 
@@ -607,7 +615,7 @@ hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
 hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st) {
     update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
         pb.r, p_of_step(pb, step), pb.p, pb.n, pb.part.rr[step & 1], rho_source(pb, step),
-        pb.vec_blocks, pb.sc);
+        pb.vec_blocks, pb.sc, pb.minv);
     return hipGetLastError();
 }
 

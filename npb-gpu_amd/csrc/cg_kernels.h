@@ -61,6 +61,7 @@ struct Fp64Problem {
     int spmv_blocks;            // workgroups of the CSR SpMV kernels
     int spmv_variant;
     int fuse_p_update;          // SELL path: p update folded into the next SpMV
+    const double *minv;         // diagonal preconditioner M^-1 (nullptr: plain CG)
     SellDev sell;
 };
 
@@ -121,6 +122,7 @@ hipError_t enqueue_solve_steps(const Fp64Problem &pb, int steps, int sets_tol,
 // FP32 side of the mixed-precision solve (cg_mixed.hip).
 struct MixedProblem {
     const float *val = nullptr;   // FP32 values in the SELL order of the FP64 copy
+    const float *minv = nullptr;  // FP32 diagonal preconditioner (nullptr: none)
     float *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;  // n+2 each
     float *rows_part = nullptr;   // nblk * n per-block row sums
     double *part_xx = nullptr;    // r0.r0 partials of an inner start [vec_blocks]
@@ -139,6 +141,10 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
 hipError_t launch_refine(const Fp64Problem &pb, const MixedProblem &mp, hipStream_t st);
 // r = b - r (r held A z), p = r, r.r partials into part.xx.
 hipError_t launch_residual_restart(const Fp64Problem &pb, hipStream_t st);
+
+hipError_t launch_narrow(const double *in, float *out, int n, int blocks, hipStream_t st);
+// minv = 1/diag(A) from the CSR.
+hipError_t launch_jacobi(const Fp64Problem &pb, double *minv, hipStream_t st);
 
 size_t scalars_bytes();
 

@@ -95,14 +95,15 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
 __global__ __launch_bounds__(kBlock) void init32_kernel(
     const double *__restrict__ b, const double *__restrict__ ax, int first, int n,
     float *__restrict__ d, float *__restrict__ r, float *__restrict__ p,
-    double *__restrict__ rr_out) {
+    double *__restrict__ rr_out, const float *__restrict__ minv) {
     double acc = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         const float rj = (float)(first ? b[j] : b[j] - ax[j]);
+        const float sj = minv ? minv[j] * rj : rj;  // s = M^-1 r
         d[j] = 0.0f;
         r[j] = rj;
-        p[j] = rj;
-        acc += (double)rj * (double)rj;
+        p[j] = sj;
+        acc += (double)rj * (double)sj;
     }
     store_block_partial<kBlock>(acc, rr_out);
 }
@@ -114,7 +115,8 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
     const float *__restrict__ p, float *__restrict__ q, float *__restrict__ d,
     float *__restrict__ r, int n, const double *__restrict__ rho_part,
     const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
-    const float *__restrict__ rows_part, int nblk, Scalars *sc, int first_step) {
+    const float *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
+    const float *__restrict__ minv) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rho_part, gridDim.x);
     const double dd = reduce_partials<kBlock>(pq_part, n_pq);
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
         sc->d = dd;
         sc->alpha = alpha;
         if (first_step) {
-            sc->tol2 = sc->rtol2 * rho;
+            sc->tol2 = sc->rtol2 * fabs(rho);
             sc->bb = rho;
         }
         sc->steps += 1;
@@ -138,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
         d[j] = fmaf(a32, p[j], d[j]);
         const float rj = fmaf(-a32, qj, r[j]);
         r[j] = rj;
-        acc += (double)rj * (double)rj;
+        acc += (double)rj * (double)(minv ? minv[j] * rj : rj);
     }
     store_block_partial<kBlock>(acc, rr_out);
 }
@@ -146,12 +148,13 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
 // beta = (new r.r) / (old r.r) in FP64; p = r + beta p; stop test.
 __global__ __launch_bounds__(kBlock) void update_p32_kernel(
     const float *__restrict__ r, float *__restrict__ p, int n,
-    const double *__restrict__ rr_part, const double *__restrict__ rho_part, Scalars *sc) {
+    const double *__restrict__ rr_part, const double *__restrict__ rho_part, Scalars *sc,
+    const float *__restrict__ minv) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rr_part, gridDim.x);
     const double rho0 = reduce_partials<kBlock>(rho_part, gridDim.x);
     const double beta = rho / rho0;
-    const bool stop = rho <= sc->tol2;
+    const bool stop = fabs(rho) <= sc->tol2;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->rho = rho;
         sc->beta = beta;
@@ -159,8 +162,13 @@ __global__ __launch_bounds__(kBlock) void update_p32_kernel(
     }
     if (stop) return;
     const float b32 = (float)beta;
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
-        p[j] = fmaf(b32, p[j], r[j]);
+    if (minv) {
+        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+            p[j] = fmaf(b32, p[j], minv[j] * r[j]);
+    } else {
+        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+            p[j] = fmaf(b32, p[j], r[j]);
+    }
 }
 
 // x += d (FP64 accumulation of the FP32 correction).
@@ -174,15 +182,36 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(double *__restrict__ x,
 // FP64 solve from its true residual.
 __global__ __launch_bounds__(kBlock) void residual_restart_kernel(
     const double *__restrict__ b, double *__restrict__ r, double *__restrict__ p, int n,
-    double *__restrict__ rr_out) {
+    double *__restrict__ rr_out, const double *__restrict__ minv) {
     double acc = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         const double rj = b[j] - r[j];
+        const double sj = minv ? minv[j] * rj : rj;
         r[j] = rj;
-        p[j] = rj;
-        acc += rj * rj;
+        p[j] = sj;
+        acc += rj * sj;
     }
     store_block_partial<kBlock>(acc, rr_out);
+}
+
+// out = (float) in (the FP32 copy of the preconditioner).
+__global__ __launch_bounds__(kBlock) void narrow_kernel(const double *__restrict__ in,
+                                                       float *__restrict__ out, int n) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        out[j] = (float)in[j];
+}
+
+// minv = 1 / diag(A) from the CSR (Jacobi).
+__global__ __launch_bounds__(kBlock) void jacobi_kernel(const int *__restrict__ rowstr,
+                                                       const int *__restrict__ colidx,
+                                                       const double *__restrict__ a,
+                                                       double *__restrict__ minv, int n) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        double d = 0.0;
+        for (int k = rowstr[j]; k < rowstr[j + 1]; k++)
+            if (colidx[k] == j) d = a[k];
+        minv[j] = d != 0.0 ? 1.0 / d : 1.0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -210,7 +239,7 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
                            int steps, hipStream_t st) {
     const int vb = pb.vec_blocks;
     init32_kernel<<<vb, kBlock, 0, st>>>(pb.x, pb.r, first_sweep, pb.n, mp.d, mp.r, mp.p,
-                                         mp.part_xx);
+                                         mp.part_xx, mp.minv);
     hipError_t e = hipGetLastError();
     for (int it = 0; it < steps && e == hipSuccess; it++) {
         const SellDev &sd = pb.sell;
@@ -227,11 +256,11 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
         update_zr32_kernel<<<vb, kBlock, 0, st>>>(mp.p, mp.q, mp.d, mp.r, pb.n,
                                                   rho32_source(mp, it), mp.part_pq, sd.grid,
                                                   mp.part_rr[it & 1], mp.rows_part, sd.nblk,
-                                                  mp.sc, it == 0 ? 1 : 0);
+                                                  mp.sc, it == 0 ? 1 : 0, mp.minv);
         e = hipGetLastError();
         if (e != hipSuccess) break;
         update_p32_kernel<<<vb, kBlock, 0, st>>>(mp.r, mp.p, pb.n, mp.part_rr[it & 1],
-                                                 rho32_source(mp, it), mp.sc);
+                                                 rho32_source(mp, it), mp.sc, mp.minv);
         e = hipGetLastError();
     }
     return e;
@@ -244,7 +273,17 @@ hipError_t launch_refine(const Fp64Problem &pb, const MixedProblem &mp, hipStrea
 
 hipError_t launch_residual_restart(const Fp64Problem &pb, hipStream_t st) {
     residual_restart_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.r, pb.p, pb.n,
-                                                              pb.part.xx);
+                                                              pb.part.xx, pb.minv);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow(const double *in, float *out, int n, int blocks, hipStream_t st) {
+    narrow_kernel<<<blocks, kBlock, 0, st>>>(in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi(const Fp64Problem &pb, double *minv, hipStream_t st) {
+    jacobi_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.rowstr, pb.colidx, pb.a, minv, pb.n);
     return hipGetLastError();
 }
 

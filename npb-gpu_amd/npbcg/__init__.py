@@ -51,19 +51,21 @@ class Opts(ctypes.Structure):
 
 
 SOLVE_FP64, SOLVE_MIXED = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_LEARNED = 0, 1, 2
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int),
                 ("sweeps", ctypes.c_int), ("fp64_spmvs", ctypes.c_int),
-                ("fp32_spmvs", ctypes.c_int), ("reserved_i", ctypes.c_int * 3),
+                ("fp32_spmvs", ctypes.c_int), ("precond", ctypes.c_int),
+                ("fallback", ctypes.c_int), ("reserved_i", ctypes.c_int),
                 ("relres", ctypes.c_double), ("time_ms", ctypes.c_double),
                 ("reserved_d", ctypes.c_double * 4)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k in ("converged", "iterations", "sweeps",
-                                               "fp64_spmvs", "fp32_spmvs", "relres",
-                                               "time_ms")}
+                                               "fp64_spmvs", "fp32_spmvs", "precond",
+                                               "fallback", "relres", "time_ms")}
 
 
 class Timing(ctypes.Structure):
@@ -107,6 +109,8 @@ def _load() -> ctypes.CDLL:
         "npbcg_solve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(SolveStats)]),
         "npbcg_get_solution": (ctypes.c_int, [vp, _F64P]),
+        "npbcg_set_preconditioner": (ctypes.c_int, [vp, ctypes.c_int]),
+        "npbcg_get_preconditioner": (ctypes.c_int, [vp, _F64P]),
         "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
         "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
                                           _F64P, _F64P, _DP]),
@@ -286,6 +290,15 @@ class Context:
         x = np.zeros(self.n)
         _check(LIB.npbcg_get_solution(self._h, x), "npbcg_get_solution")
         return x
+
+    def set_preconditioner(self, kind: int):
+        _check(LIB.npbcg_set_preconditioner(self._h, kind), "npbcg_set_preconditioner")
+
+    def preconditioner(self) -> np.ndarray:
+        """The active M^-1 diagonal (ones for plain CG)."""
+        m = np.zeros(self.n)
+        _check(LIB.npbcg_get_preconditioner(self._h, m), "npbcg_get_preconditioner")
+        return m
 
     def copy_bench(self, nbytes: int, reps: int = 10) -> float:
         """Mean ms of a device copy of nbytes (read nbytes + write nbytes)."""

@@ -68,6 +68,9 @@ def _lib() -> ctypes.CDLL:
         lib.orc_cg_solve.restype = ctypes.c_int
         lib.orc_cg_solve.argtypes = [ctypes.c_int, _I32P, _I32P, _F64P, _F64P,
                                      _F64P, ctypes.c_double, ctypes.c_int, _DP, _F64P]
+        lib.orc_pcg_solve.restype = ctypes.c_int
+        lib.orc_pcg_solve.argtypes = [ctypes.c_int, _I32P, _I32P, _F64P, _F64P, _F64P,
+                                      _F64P, ctypes.c_double, ctypes.c_int, _DP, _F64P]
         lib.orc_make_rhs.restype = None
         lib.orc_make_rhs.argtypes = [ctypes.c_int, ctypes.c_double, _F64P]
         _LIB = lib
@@ -170,6 +173,29 @@ def cg_solve(m: CSR, b: np.ndarray, rtol: float = 1e-10, maxit: int = 10000):
                              np.ascontiguousarray(b, dtype=np.float64), x, rtol,
                              maxit, ctypes.byref(rr), work)
     return x, it, rr.value
+
+
+def pcg_solve(m: CSR, b: np.ndarray, minv: np.ndarray, rtol: float = 1e-10,
+              maxit: int = 10000):
+    """FP64 CG preconditioned by the diagonal minv.  Returns (x, iters, relres)."""
+    n = m.n
+    x = np.zeros(n)
+    work = np.zeros(5 * n)
+    rr = ctypes.c_double(0.0)
+    it = _lib().orc_pcg_solve(n, m.rowstr, m.colidx, m.a,
+                              np.ascontiguousarray(minv, dtype=np.float64),
+                              np.ascontiguousarray(b, dtype=np.float64), x, rtol,
+                              maxit, ctypes.byref(rr), work)
+    return x, it, rr.value
+
+
+def diagonal(m: CSR) -> np.ndarray:
+    """a_jj of every row (0 where the row stores no diagonal entry)."""
+    d = np.zeros(m.n)
+    rows = np.repeat(np.arange(m.n), np.diff(m.rowstr[: m.n + 1]))
+    on = m.colidx[: m.rowstr[m.n]] == rows
+    d[rows[on]] = m.a[: m.rowstr[m.n]][on]
+    return d
 
 
 # ---------------------------------------------------------------------------

@@ -339,6 +339,63 @@ int orc_cg_solve(int n, const int *rowstr, const int *colidx, const double *a,
     return it;
 }
 
+/* Diagonal-preconditioned CG (PCG, M^-1 = diag(minv)): z = M^-1 r,
+ * rho = r.z, p = z + beta p.  The true residual is checked when
+ * |rho| <= rtol^2 |rho_0| (the preconditioned norm test) and every 25 steps;
+ * the loop ends when ||b - A x|| <= rtol ||b||.  work holds 5n doubles. */
+int orc_pcg_solve(int n, const int *rowstr, const int *colidx, const double *a,
+                  const double *minv, const double *b, double *x, double rtol,
+                  int maxit, double *relres, double *work)
+{
+    double *r = work, *p = work + n, *q = work + 2 * (size_t)n,
+           *t = work + 3 * (size_t)n, *z = work + 4 * (size_t)n;
+    const int check = 25;
+    double bnorm = sqrt(dot_n(n, b, b));
+    for (int j = 0; j < n; j++) {
+        x[j] = 0.0;
+        r[j] = b[j];
+        z[j] = minv[j] * b[j];
+        p[j] = z[j];
+    }
+    if (bnorm == 0.0) {
+        *relres = 0.0;
+        return 0;
+    }
+    double rho = dot_n(n, r, z);
+    const double rho_first = fabs(rho);
+    int it = 0;
+    double true_rel = 1.0;
+    while (it < maxit) {
+        orc_spmv(n, rowstr, colidx, a, p, q);
+        double d = dot_n(n, p, q);
+        double alpha = rho / d;
+        double rho0 = rho;
+        for (int j = 0; j < n; j++) {
+            x[j] = x[j] + alpha * p[j];
+            r[j] = r[j] - alpha * q[j];
+            z[j] = minv[j] * r[j];
+        }
+        rho = dot_n(n, r, z);
+        it++;
+        if (fabs(rho) <= rtol * rtol * rho_first || it % check == 0) {
+            orc_spmv(n, rowstr, colidx, a, x, t);
+            double s = 0.0;
+            for (int j = 0; j < n; j++) {
+                double e = b[j] - t[j];
+                s = s + e * e;
+            }
+            true_rel = sqrt(s) / bnorm;
+            if (true_rel <= rtol)
+                break;
+        }
+        double beta = rho / rho0;
+        for (int j = 0; j < n; j++)
+            p[j] = z[j] + beta * p[j];
+    }
+    *relres = true_rel;
+    return it;
+}
+
 void orc_make_rhs(int n, double seed, double *b)
 {
     double t = seed + 1.0;

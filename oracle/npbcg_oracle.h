@@ -83,6 +83,10 @@ int orc_cg_solve(int n, const int *rowstr, const int *colidx, const double *a,
  * b[i] = 2*u_i - 1 with u_i successive orc_randlc draws from tran = seed+1
  * (no discarded draw).  The reference has no such vector; this is the
  * build's own definition, recorded in the golden fixtures. */
+/* Diagonal-preconditioned CG; work holds 5n doubles. */
+int orc_pcg_solve(int n, const int *rowstr, const int *colidx, const double *a,
+                  const double *minv, const double *b, double *x, double rtol,
+                  int maxit, double *relres, double *work);
 void orc_make_rhs(int n, double seed, double *b);
 
 #ifdef __cplusplus

@@ -46,7 +46,7 @@ def test_solve_small(gpu, mode, n, nonzer, shift, seed):
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_solve_cfg3(gpu, mode):
-    # n=150000, nonzer=15, shift=110, seed 314, b = 2u-1.
+    # The solve workload: n=150000, nonzer=15, shift=110, seed 314, b = 2u-1.
     m = gpu.makea(150000, 15, 110.0, 314.0)
     b = oracle.make_rhs(m.n, 314.0)
     with gpu.Context(m) as ctx:
@@ -96,3 +96,35 @@ def test_solve_leaves_power_state(gpu, golden):
         z2, _ = ctx.power(3, c["shift"])   # continues from the x left by the first run
     ref = [float.fromhex(h) for h in c["zeta_hex"][:6]]
     assert np.all(np.abs(np.concatenate([z1, z2]) - ref) / np.abs(ref) <= 1e-12)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_solve_jacobi_pcg(gpu, mode):
+    m = gpu.makea(20000, 11, 30.0, 271.0)
+    om = _ocsr(m)
+    b = oracle.make_rhs(m.n, 271.0)
+    dinv = 1.0 / oracle.diagonal(om)
+    with gpu.Context(m, spmv_variant=5) as ctx:
+        assert np.all(ctx.preconditioner() == 1.0)          # plain CG by default
+        with pytest.raises(gpu.NpbcgError) as e:
+            ctx.set_preconditioner(gpu.PRECOND_LEARNED)      # no learned diagonal yet
+        assert e.value.code == gpu.ERR_INVALID
+        ctx.set_preconditioner(gpu.PRECOND_JACOBI)
+        minv = ctx.preconditioner()
+        ctx.set_rhs(b)
+        st = ctx.solve(mode, RTOL, 2000)
+        x = ctx.solution()
+        ctx.set_preconditioner(gpu.PRECOND_NONE)
+        st0 = ctx.solve(mode, RTOL, 2000)
+    # 1/a_jj is one correctly rounded division on both sides
+    assert np.array_equal(minv, dinv)
+    rel = _true_relres(m, x, b)
+    ox, oit, orel = oracle.pcg_solve(om, b, dinv, rtol=RTOL, maxit=5000)
+    err = float(np.max(np.abs(x - ox)) / np.max(np.abs(ox)))
+    print(f"jacobi mode {mode}: {st.as_dict()} host relres {rel:.2e}; oracle PCG its {oit}; "
+          f"plain {st0.iterations}; x rel err {err:.2e}")
+    assert st.converged == 1 and rel <= RTOL and st.precond == gpu.PRECOND_JACOBI
+    assert st.fallback == 0 and st0.precond == gpu.PRECOND_NONE
+    assert err <= 1e-8
+    if mode == 0:   # same recurrence as the oracle's PCG: step counts agree closely
+        assert abs(st.iterations - oit) <= 2

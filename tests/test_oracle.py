@@ -137,6 +137,28 @@ def test_cg_solve_reaches_true_residual():
     assert 0 < it < 1000
 
 
+def test_pcg_solve_reduces_to_cg_and_solves_with_diagonal():
+    m = _tiny(n=64, nonzer=4, shift=6.0, seed=271.0)
+    b = oracle.make_rhs(m.n, 271.0)
+    d = m.dense()
+    assert np.array_equal(oracle.diagonal(m), np.diag(d))
+    x0, it0, rel0 = oracle.cg_solve(m, b, rtol=1e-12, maxit=1000)
+    x1, it1, rel1 = oracle.pcg_solve(m, b, np.ones(m.n), rtol=1e-12, maxit=1000)
+    # M = I is plain CG, step for step
+    assert np.array_equal(x0, x1) and it0 == it1 and rel0 == rel1
+    # Jacobi and a positive random diagonal both reach the true residual
+    rng = np.random.default_rng(5)
+    for minv in (1.0 / np.diag(d), np.exp(rng.uniform(-1, 1, m.n)) / np.diag(d)):
+        x, it, rel = oracle.pcg_solve(m, b, minv, rtol=1e-12, maxit=1000)
+        assert rel <= 1e-12 and 0 < it < 1000
+        assert np.linalg.norm(b - d @ x) / np.linalg.norm(b) <= 1e-12
+    # With M^-1 = A^-1 (the diagonal of a diagonal matrix) PCG takes one step.
+    dm = oracle.CSR(m.n, np.arange(m.n + 1, dtype=np.int32), np.arange(m.n, dtype=np.int32),
+                    np.diag(d).copy(), m.ratio, m.nonzer, m.shift, m.rcond, m.seed)
+    x, it, rel = oracle.pcg_solve(dm, b, 1.0 / np.diag(d), rtol=1e-12, maxit=10)
+    assert it == 1 and rel <= 1e-15
+
+
 def test_make_rhs_definition():
     b = oracle.make_rhs(5, 314.0)
     t = 315.0
