@@ -189,20 +189,46 @@ SOLVE_WORKLOAD = dict(workload="npb_cg_solve_n150000", n=150000, nonzer=15, shif
                       rcond=0.1, seed=314.0, rtol=1e-10)
 
 
+LEARNED_PARAMS = os.path.join(ROOT, "npb-gpu_amd", "npbcg", "learned_precond.npz")
+
+
 def solve_comparison(reps: int):
     """Time to a 1e-10 FP64-checked relative residual on the n=150000 matrix
-    (b = 2u - 1 from the LCG stream seeded 315): plain FP64 CG vs FP64-refined
-    mixed precision, each the median of `reps` solves with b resident."""
+    (b = 2u - 1 from the LCG stream seeded 315), each variant the median of
+    `reps` solves with b resident: plain FP64 CG, FP64-refined mixed
+    precision, and mixed precision preconditioned by the top-K Neumann form
+    (g = h = 0) and by the trained network (when its parameter file exists).
+    Preconditioner set-up (graph, network inference, operator values) is
+    timed separately: it is paid once per matrix, not per solve."""
     import npbcg
 
     sw = SOLVE_WORKLOAD
     m = npbcg.makea(sw["n"], sw["nonzer"], sw["shift"], sw["seed"], sw["rcond"])
     b = rhs_for(sw["n"], sw["seed"])
     out = {"workload": sw["workload"], "n": sw["n"], "nnz": m.nnz, "rtol": sw["rtol"]}
+    variants = [("fp64", npbcg.SOLVE_FP64, npbcg.PRECOND_NONE),
+                ("mixed", npbcg.SOLVE_MIXED, npbcg.PRECOND_NONE),
+                ("fp64_neumann", npbcg.SOLVE_FP64, npbcg.PRECOND_NEUMANN),
+                ("mixed_neumann", npbcg.SOLVE_MIXED, npbcg.PRECOND_NEUMANN)]
+    net = None
+    if os.path.exists(LEARNED_PARAMS):
+        variants += [("fp64_learned", npbcg.SOLVE_FP64, npbcg.PRECOND_LEARNED),
+                     ("mixed_learned", npbcg.SOLVE_MIXED, npbcg.PRECOND_LEARNED)]
     with npbcg.Context(m, device=0) as ctx:
         ctx.set_rhs(b)
-        for name, mode in (("fp64", npbcg.SOLVE_FP64), ("mixed", npbcg.SOLVE_MIXED)):
-            ctx.solve(mode, sw["rtol"], 500)  # warm
+        t0 = time.perf_counter()
+        ctx.set_preconditioner(npbcg.PRECOND_NEUMANN)
+        ctx.preconditioner()                       # builds graph and operator
+        out["precond_setup_wall_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        if os.path.exists(LEARNED_PARAMS):
+            net = npbcg.Net(seed=1)
+            net.load(LEARNED_PARAMS)
+            net.apply(ctx)                         # warm
+            inf = [net.apply(ctx) for _ in range(5)]
+            out["learned_inference_ms_median"] = round(float(np.median(inf)), 4)
+        for name, mode, kind in variants:
+            ctx.set_preconditioner(kind)
+            ctx.solve(mode, sw["rtol"], 500)  # warm (also loads the operator values)
             ev, wall, st = [], [], None
             for _ in range(reps):
                 t0 = time.perf_counter()
@@ -213,8 +239,11 @@ def solve_comparison(reps: int):
             d["time_ms_median"] = round(float(np.median(ev)), 4)
             d["wall_ms_median"] = round(float(np.median(wall)), 4)
             out[name] = d
-    out["mixed_speedup_vs_fp64"] = round(out["fp64"]["time_ms_median"] /
-                                         out["mixed"]["time_ms_median"], 3)
+    if net is not None:
+        net.close()
+    base = out["fp64"]["time_ms_median"]
+    out["speedup_vs_fp64"] = {k: round(base / out[k]["time_ms_median"], 3)
+                              for k, _, _ in variants if k != "fp64"}
     return out
 
 

@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define NPBCG_ABI_VERSION 1
+#define NPBCG_ABI_VERSION 2
 
 typedef enum npbcg_status {
     NPBCG_OK = 0,
@@ -184,14 +184,21 @@ int npbcg_copy_bench(npbcg_ctx *ctx, long long bytes, int reps, double *avg_ms);
  *                      Needs the LDS-staged layout (spmv_variant 5). */
 enum { NPBCG_SOLVE_FP64 = 0, NPBCG_SOLVE_MIXED = 1 };
 
-/* Diagonal preconditioners M^-1 for npbcg_solve (PCG: rho = r.M^-1 r,
- * p = M^-1 r + beta p; the stop test uses |r.M^-1 r| <= rtol^2 |r0.M^-1 r0|,
- * the final check is always the FP64 true residual). */
+/* Preconditioners M^-1 for npbcg_solve (PCG: rho = r.M^-1 r, p = M^-1 r +
+ * beta p; the stop test uses |r.M^-1 r| <= rtol^2 |r0.M^-1 r0|, the final
+ * check is always the FP64 true residual).  NEUMANN and LEARNED share one
+ * sparse form on the K = 4 largest off-diagonals |a_jk| of every row:
+ *   M^-1 = S D^-1 S - Q (N + N^T) Q,  N_jk = a_jk / (a_jj a_kk),
+ *   S = diag(exp g), Q = diag(exp(g + h)),
+ * applied as one SELL-64 product per CG step. */
 enum {
     NPBCG_PRECOND_NONE = 0,     /* plain CG */
     NPBCG_PRECOND_JACOBI = 1,   /* M^-1 = 1 / diag(A) */
-    NPBCG_PRECOND_LEARNED = 2   /* M^-1 = exp(g) / diag(A), g from npbcg_net_apply;
-                                   falls back to Jacobi if the solve fails */
+    NPBCG_PRECOND_LEARNED = 2,  /* g, h per row from npbcg_net_apply; falls back
+                                   to Jacobi-CG if the solve fails */
+    NPBCG_PRECOND_NEUMANN = 3   /* g = h = 0: D^-1 - D^-1 (E + E^T) D^-1, E the top-K
+                                   off-diagonals (a Neumann-type approximation
+                                   of A^-1 and the network's base) */
 };
 
 typedef struct npbcg_solve_stats {
@@ -216,11 +223,86 @@ int npbcg_solve(npbcg_ctx *ctx, int mode, double rtol, int maxit,
                 npbcg_solve_stats *stats);
 /* Downloads the last solution (n entries). */
 int npbcg_get_solution(npbcg_ctx *ctx, double *x);
-/* Selects the preconditioner of later solves.  LEARNED needs a diagonal from
- * npbcg_net_apply (NPBCG_ERR_INVALID otherwise). */
+/* Selects the preconditioner of later solves.  LEARNED needs npbcg_net_apply
+ * on this context first (NPBCG_ERR_INVALID otherwise). */
 int npbcg_set_preconditioner(npbcg_ctx *ctx, int kind);
-/* Downloads the active M^-1 diagonal (n entries; plain CG: ones). */
+/* Downloads the diagonal of the active M^-1 (n entries; plain CG: ones). */
 int npbcg_get_preconditioner(npbcg_ctx *ctx, double *minv);
+/* Applies the active M^-1 to v (n entries, host) into z: z = M^-1 v in FP64. */
+int npbcg_apply_preconditioner(npbcg_ctx *ctx, const double *v, double *z);
+
+/* ---- learned preconditioner ------------------------------------------------
+ * Training data are residual/correction pairs the solver generates itself:
+ * plain FP64 CG on A x = b to x* (rtol 1e-12), then, from x_0 = 0, the CG
+ * residuals r_k and corrections e_k = x* - x_k of its first P steps.  The
+ * network (3 message-passing layers over the top-K row graph, hidden 64,
+ * BF16 MFMA with FP32 accumulation) predicts g, h per row and is trained with
+ * Adam on L = sum_k |M^-1 r_k - e_k|^2 / (P |e_k|^2); npbcg_net_train weights
+ * each system's gradient by 1 / L(g = h = 0), the loss of the Neumann base. */
+#define NPBCG_MAX_PAIRS 8
+#define NPBCG_NET_FEATURES 8
+
+/* Plain CG to x* for the resident right-hand side, then the first `pairs`
+ * residual/correction pairs, kept on the device for npbcg_net_loss_grad;
+ * r and e (pairs * n floats each, pair-major) receive copies when not NULL. */
+int npbcg_log_pairs(npbcg_ctx *ctx, int pairs, float *r, float *e);
+/* Replaces the device pairs with host ones (pairs * n floats each). */
+int npbcg_set_pairs(npbcg_ctx *ctx, int pairs, const float *r, const float *e);
+/* The top-K graph of the context's matrix: neighbours nbr[n*K], message
+ * weights wgt[n*K], N values nval[n*K] and features feat[n*8] (any may be
+ * NULL); K is returned. */
+int npbcg_graph(npbcg_ctx *ctx, int *nbr, float *wgt, float *nval, float *feat);
+
+typedef struct npbcg_net npbcg_net;
+
+typedef struct npbcg_train_opts {
+    int systems;             /* training systems */
+    int n_min, n_max;        /* order n, uniform */
+    int nonzer_min, nonzer_max;   /* nonzeros per generator vector, uniform */
+    double shift_min, shift_max;  /* diagonal shift, log-uniform */
+    double rcond;
+    unsigned long long seed; /* seeds every system's (n, nonzer, shift, matrix seed, b) */
+    int pairs;               /* residual/correction pairs per system (<= 8) */
+    int steps_per_system;    /* Adam steps per system */
+    float lr;                /* Adam learning rate */
+    int gen_threads;         /* host generator threads (0: up to 16) */
+    int first_system;        /* index of the first system (runs may be split in chunks) */
+    int reserved[5];
+} npbcg_train_opts;
+
+typedef struct npbcg_train_stats {
+    int systems;
+    int steps;
+    double loss_first;       /* mean loss of the first 10% of systems, before their step */
+    double loss_last;        /* same, last 10% */
+    double base_first;       /* Neumann (g = h = 0) loss of the same systems */
+    double base_last;
+    double wait_ms;          /* time the device side waited for the generator */
+    double total_ms;
+    double reserved[4];
+} npbcg_train_stats;
+
+int npbcg_net_param_count(void);
+/* A network with He-initialised layers and a zero readout (g = h = 0). */
+int npbcg_net_create(npbcg_net **net, int device, unsigned long long seed);
+void npbcg_net_destroy(npbcg_net *net);
+/* Parameters (npbcg_net_param_count floats) and feature normalisation
+ * (2 * NPBCG_NET_FEATURES floats: mean then std; all-zero if never set). */
+int npbcg_net_get_params(npbcg_net *net, float *params, float *feature_norm);
+int npbcg_net_set_params(npbcg_net *net, const float *params, const float *feature_norm);
+void npbcg_net_default_train_opts(npbcg_train_opts *opts);
+/* Trains on opts->systems generated systems.  loss_hist and base_hist (may be
+ * NULL) receive, per system, the loss before its step and the Neumann loss. */
+int npbcg_net_train(npbcg_net *net, const npbcg_train_opts *opts, npbcg_train_stats *stats,
+                    double *loss_hist, double *base_hist);
+/* Loss of the context's resident pairs; grad (param_count floats) and gh
+ * (g then h, 2n floats) when not NULL.  Does not change the parameters. */
+int npbcg_net_loss_grad(npbcg_net *net, npbcg_ctx *ctx, double *loss, float *grad, float *gh);
+/* One Adam step on the gradient of the context's resident pairs. */
+int npbcg_net_step(npbcg_net *net, npbcg_ctx *ctx, float lr, double *loss);
+/* Inference: g, h for the context's matrix, installed as its LEARNED
+ * preconditioner; ms (may be NULL) receives the device time. */
+int npbcg_net_apply(npbcg_net *net, npbcg_ctx *ctx, double *ms);
 
 /* ---- reference-shaped shim ------------------------------------------------
  * For a caller that keeps the reference's exact call

@@ -13,6 +13,7 @@
 
 #include "cg_device.h"
 #include "cg_kernels.h"
+#include "learned_kernels.h"
 #include "npbcg.h"
 #include "npbcg_internal.h"
 #include "tiled_layout.h"
@@ -99,11 +100,40 @@ struct npbcg_ctx {
     float *d_vec32 = nullptr;      // d, r, p, q (FP32), stride vstride
     int precond = NPBCG_PRECOND_NONE;
     bool learned_ready = false;
-    double *d_minv = nullptr;      // [0, n) Jacobi, [vstride, ...) learned (FP64)
-    float *d_minv32 = nullptr;     // FP32 copy of the active diagonal
+    double *d_minv = nullptr;      // Jacobi diagonal (FP64)
+    float *d_minv32 = nullptr;     // its FP32 copy
     float *d_rows32 = nullptr;
     double *d_part32 = nullptr;
     Scalars *d_sc32 = nullptr;
+
+    // top-K graph and the sparse preconditioner operator (lazy)
+    struct Graph {
+        bool built = false;
+        double *adiag = nullptr;
+        float *feat = nullptr;
+        int *nbr = nullptr;
+        float *wgt = nullptr, *nval = nullptr;
+        int *tptr = nullptr, *tsrc = nullptr;
+        float *tw = nullptr, *tval = nullptr;
+        int *cnt = nullptr;
+        int *op_off = nullptr, *op_col = nullptr;
+        float *op_src = nullptr;
+        double *op_val = nullptr;
+        float *op_val32 = nullptr;
+        float *gh_zero = nullptr, *gh_learned = nullptr;  // (g, h), 2n each
+        double *pz = nullptr;    // z = M^-1 r work vectors
+        float *pz32 = nullptr;
+        std::vector<int> h_off;
+        npbcg_dev::PrecondOp op;
+        int loaded = -1;         // preconditioner kind whose values op holds
+    } g;
+
+    // residual/correction pairs (lazy)
+    struct Pairs {
+        int count = 0;
+        float *r = nullptr, *e = nullptr, *omega = nullptr;
+        double *part = nullptr, *xstar = nullptr;
+    } pairs;
 };
 
 // Device vectors: x, z, p, q, r of the reference plus p_alt (fused p
@@ -529,6 +559,14 @@ void npbcg_destroy(npbcg_ctx *c) {
     (void)hipFree(c->d_val32);
     (void)hipFree(c->d_minv);
     (void)hipFree(c->d_minv32);
+    for (void *p : {(void *)c->g.adiag, (void *)c->g.feat, (void *)c->g.nbr, (void *)c->g.wgt,
+                    (void *)c->g.nval, (void *)c->g.tptr, (void *)c->g.tsrc, (void *)c->g.tw,
+                    (void *)c->g.tval, (void *)c->g.cnt, (void *)c->g.op_off, (void *)c->g.op_col,
+                    (void *)c->g.op_src, (void *)c->g.op_val, (void *)c->g.op_val32,
+                    (void *)c->g.gh_zero, (void *)c->g.gh_learned, (void *)c->g.pz,
+                    (void *)c->g.pz32, (void *)c->pairs.r, (void *)c->pairs.e,
+                    (void *)c->pairs.omega, (void *)c->pairs.part, (void *)c->pairs.xstar})
+        (void)hipFree(p);
     (void)hipFree(c->d_vec32);
     (void)hipFree(c->d_rows32);
     (void)hipFree(c->d_part32);
@@ -870,26 +908,158 @@ namespace {
 
 int ensure_precond_buffers(npbcg_ctx *c) {
     if (c->d_minv) return NPBCG_OK;
-    NPBCG_HIP(hipMalloc(&c->d_minv, sizeof(double) * 2 * c->vstride));
+    NPBCG_HIP(hipMalloc(&c->d_minv, sizeof(double) * c->vstride));
     NPBCG_HIP(hipMalloc(&c->d_minv32, sizeof(float) * c->vstride));
     NPBCG_HIP(npbcg_dev::launch_jacobi(c->pb, c->d_minv, c->stream));
+    NPBCG_HIP(npbcg_dev::launch_narrow(c->d_minv, c->d_minv32, c->n, c->pb.vec_blocks,
+                                       c->stream));
     NPBCG_HIP(hipStreamSynchronize(c->stream));
     return NPBCG_OK;
 }
 
-// Points the solve views at the diagonal of `kind` (FP64 and its FP32 copy).
+// The top-K graph (device) and the SELL-64 layout of the sparse operator
+// M^-1: row j holds its diagonal, its K out-neighbours and its in-neighbours
+// (rows whose top-K contain j), padded to the longest row of its slice.
+int ensure_row_graph(npbcg_ctx *c) {
+    auto &g = c->g;
+    if (g.built) return NPBCG_OK;
+    const int n = c->n;
+    const int K = npbcg_dev::kNbr;
+    const size_t nk = (size_t)n * K;
+    NPBCG_HIP(hipMalloc(&g.adiag, sizeof(double) * n));
+    NPBCG_HIP(hipMalloc(&g.feat, sizeof(float) * n * npbcg_dev::kFeat));
+    NPBCG_HIP(hipMalloc(&g.nbr, sizeof(int) * nk));
+    NPBCG_HIP(hipMalloc(&g.wgt, sizeof(float) * nk));
+    NPBCG_HIP(hipMalloc(&g.nval, sizeof(float) * nk));
+    NPBCG_HIP(hipMalloc(&g.tptr, sizeof(int) * (n + 1)));
+    NPBCG_HIP(hipMalloc(&g.tsrc, sizeof(int) * nk));
+    NPBCG_HIP(hipMalloc(&g.tw, sizeof(float) * nk));
+    NPBCG_HIP(hipMalloc(&g.tval, sizeof(float) * nk));
+    NPBCG_HIP(hipMalloc(&g.cnt, sizeof(int) * n));
+    NPBCG_HIP(npbcg_dev::launch_graph_features(c->d_rowstr, c->d_colidx, c->d_a, n, g.adiag,
+                                               g.feat, g.nbr, g.wgt, g.nval, c->stream));
+    NPBCG_HIP(npbcg_dev::launch_transpose(g.nbr, g.wgt, g.nval, n, g.cnt, g.tptr, g.tsrc, g.tw,
+                                          g.tval, c->stream));
+    std::vector<int> nbr(nk), tptr(n + 1);
+    std::vector<float> nval(nk);
+    NPBCG_HIP(hipMemcpyAsync(nbr.data(), g.nbr, sizeof(int) * nk, hipMemcpyDeviceToHost,
+                             c->stream));
+    NPBCG_HIP(hipMemcpyAsync(nval.data(), g.nval, sizeof(float) * nk, hipMemcpyDeviceToHost,
+                             c->stream));
+    NPBCG_HIP(hipMemcpyAsync(tptr.data(), g.tptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost,
+                             c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    const int edges = tptr[n];
+    std::vector<int> tsrc(std::max(edges, 1));
+    std::vector<float> tval(std::max(edges, 1));
+    if (edges > 0) {
+        NPBCG_HIP(hipMemcpyAsync(tsrc.data(), g.tsrc, sizeof(int) * edges, hipMemcpyDeviceToHost,
+                                 c->stream));
+        NPBCG_HIP(hipMemcpyAsync(tval.data(), g.tval, sizeof(float) * edges,
+                                 hipMemcpyDeviceToHost, c->stream));
+        NPBCG_HIP(hipStreamSynchronize(c->stream));
+    }
+    const int W = npbcg_dev::kWave;
+    const int slices = (n + W - 1) / W;
+    g.h_off.assign(slices + 1, 0);
+    for (int s = 0; s < slices; s++) {
+        int len = 1;
+        for (int j = s * W; j < std::min(n, (s + 1) * W); j++)
+            len = std::max(len, 1 + K + tptr[j + 1] - tptr[j]);
+        g.h_off[s + 1] = g.h_off[s] + len * W;
+    }
+    const size_t slots = (size_t)g.h_off[slices];
+    std::vector<int> col(slots, 0);
+    std::vector<float> src(slots, 0.0f);
+    for (int s = 0; s < slices; s++) {
+        const int len = (g.h_off[s + 1] - g.h_off[s]) / W;
+        for (int r = 0; r < W; r++) {
+            const int j = s * W + r;
+            if (j >= n) break;
+            auto slot = [&](int t) { return (size_t)g.h_off[s] + (size_t)t * W + r; };
+            for (int t = 0; t < len; t++) col[slot(t)] = j;  // padding reads r_j, value 0
+            for (int t = 0; t < K; t++) {
+                col[slot(1 + t)] = nbr[(size_t)j * K + t];
+                src[slot(1 + t)] = nval[(size_t)j * K + t];
+            }
+            for (int e = tptr[j]; e < tptr[j + 1]; e++) {
+                col[slot(1 + K + e - tptr[j])] = tsrc[e];
+                src[slot(1 + K + e - tptr[j])] = tval[e];
+            }
+        }
+    }
+    NPBCG_HIP(hipMalloc(&g.op_off, sizeof(int) * (slices + 1)));
+    NPBCG_HIP(hipMalloc(&g.op_col, sizeof(int) * slots));
+    NPBCG_HIP(hipMalloc(&g.op_src, sizeof(float) * slots));
+    NPBCG_HIP(hipMalloc(&g.op_val, sizeof(double) * slots));
+    NPBCG_HIP(hipMalloc(&g.op_val32, sizeof(float) * slots));
+    NPBCG_HIP(hipMalloc(&g.gh_zero, sizeof(float) * 2 * n));
+    NPBCG_HIP(hipMalloc(&g.gh_learned, sizeof(float) * 2 * n));
+    NPBCG_HIP(hipMalloc(&g.pz, sizeof(double) * c->vstride));
+    NPBCG_HIP(hipMalloc(&g.pz32, sizeof(float) * c->vstride));
+    NPBCG_HIP(hipMemcpyAsync(g.op_off, g.h_off.data(), sizeof(int) * (slices + 1),
+                             hipMemcpyHostToDevice, c->stream));
+    NPBCG_HIP(hipMemcpyAsync(g.op_col, col.data(), sizeof(int) * slots, hipMemcpyHostToDevice,
+                             c->stream));
+    NPBCG_HIP(hipMemcpyAsync(g.op_src, src.data(), sizeof(float) * slots, hipMemcpyHostToDevice,
+                             c->stream));
+    NPBCG_HIP(hipMemsetAsync(g.gh_zero, 0, sizeof(float) * 2 * n, c->stream));
+    NPBCG_HIP(hipMemsetAsync(g.gh_learned, 0, sizeof(float) * 2 * n, c->stream));
+    NPBCG_HIP(hipMemsetAsync(g.pz, 0, sizeof(double) * c->vstride, c->stream));
+    NPBCG_HIP(hipMemsetAsync(g.pz32, 0, sizeof(float) * c->vstride, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));  // host vectors go out of scope
+    g.op.n = n;
+    g.op.slices = slices;
+    g.op.off = g.op_off;
+    g.op.col = g.op_col;
+    g.op.src_val = g.op_src;
+    g.op.val = g.op_val;
+    g.op.val32 = g.op_val32;
+    g.loaded = -1;
+    g.built = true;
+    return NPBCG_OK;
+}
+
+// Points the solve views at the preconditioner of `kind`.
 int select_precond(npbcg_ctx *c, int kind) {
-    if (kind == NPBCG_PRECOND_NONE) {
-        c->pbs.minv = nullptr;
-        c->mp.minv = nullptr;
+    c->pbs.minv = nullptr;
+    c->mp.minv = nullptr;
+    c->pbs.pop = nullptr;
+    c->pbs.pz = nullptr;
+    c->mp.pop = nullptr;
+    c->mp.pz = nullptr;
+    if (kind == NPBCG_PRECOND_NONE) return NPBCG_OK;
+    if (kind == NPBCG_PRECOND_JACOBI) {
+        int rc = ensure_precond_buffers(c);
+        if (rc != NPBCG_OK) return rc;
+        c->pbs.minv = c->d_minv;
+        c->mp.minv = c->d_minv32;
         return NPBCG_OK;
     }
-    int rc = ensure_precond_buffers(c);
+    int rc = ensure_row_graph(c);
     if (rc != NPBCG_OK) return rc;
-    const double *m = kind == NPBCG_PRECOND_LEARNED ? c->d_minv + c->vstride : c->d_minv;
-    NPBCG_HIP(npbcg_dev::launch_narrow(m, c->d_minv32, c->n, c->pb.vec_blocks, c->stream));
-    c->pbs.minv = m;
-    c->mp.minv = c->d_minv32;
+    auto &g = c->g;
+    if (g.loaded != kind) {
+        const float *gh = kind == NPBCG_PRECOND_LEARNED ? g.gh_learned : g.gh_zero;
+        NPBCG_HIP(npbcg_dev::launch_precond_values(g.op, g.adiag, gh, c->stream));
+        g.loaded = kind;
+    }
+    c->pbs.pop = &g.op;
+    c->pbs.pz = g.pz;
+    c->mp.pop = &g.op;
+    c->mp.pz = g.pz32;
+    return NPBCG_OK;
+}
+
+int ensure_pairs(npbcg_ctx *c) {
+    auto &pr = c->pairs;
+    if (pr.r) return NPBCG_OK;
+    const size_t pn = (size_t)NPBCG_MAX_PAIRS * c->n;
+    NPBCG_HIP(hipMalloc(&pr.r, sizeof(float) * pn));
+    NPBCG_HIP(hipMalloc(&pr.e, sizeof(float) * pn));
+    NPBCG_HIP(hipMalloc(&pr.omega, sizeof(float) * NPBCG_MAX_PAIRS));
+    NPBCG_HIP(hipMalloc(&pr.part, sizeof(double) * NPBCG_MAX_PAIRS * c->pb.vec_blocks));
+    NPBCG_HIP(hipMalloc(&pr.xstar, sizeof(double) * c->vstride));
     return NPBCG_OK;
 }
 
@@ -901,27 +1071,61 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
 
 namespace npbcg_detail {
 
-double *ctx_learned_diag(npbcg_ctx *c) {
-    if (ensure_precond_buffers(c) != NPBCG_OK) return nullptr;
-    c->learned_ready = true;
-    return c->d_minv + c->vstride;
+// The context's matrix as the network sees it (builds the graph if needed).
+int ctx_net_system(npbcg_ctx *c, npbcg_dev::NetSystem *sys) {
+    int rc = ensure_row_graph(c);
+    if (rc != NPBCG_OK) return rc;
+    const auto &g = c->g;
+    sys->n = c->n;
+    sys->adiag = g.adiag;
+    sys->feat = g.feat;
+    sys->nbr = g.nbr;
+    sys->wgt = g.wgt;
+    sys->nval = g.nval;
+    sys->tptr = g.tptr;
+    sys->tsrc = g.tsrc;
+    sys->tw = g.tw;
+    sys->tval = g.tval;
+    return NPBCG_OK;
 }
 
-const npbcg_dev::Fp64Problem *ctx_problem(const npbcg_ctx *c) { return &c->pb; }
-const npbcg_dev::Fp64Problem *ctx_solve_view(const npbcg_ctx *c) { return &c->pbs; }
+// The resident pairs (count 0 before npbcg_log_pairs / npbcg_set_pairs).
+npbcg_dev::PairSet ctx_pairs(const npbcg_ctx *c) {
+    npbcg_dev::PairSet ps;
+    ps.count = c->pairs.count;
+    ps.r = c->pairs.r;
+    ps.e = c->pairs.e;
+    ps.omega = c->pairs.omega;
+    return ps;
+}
+
+const float *ctx_zero_gh(const npbcg_ctx *c) { return c->g.gh_zero; }
+
+// Installs (g, h) from the device array gh as the LEARNED preconditioner.
+int ctx_install_learned(npbcg_ctx *c, const float *gh) {
+    int rc = ensure_row_graph(c);
+    if (rc != NPBCG_OK) return rc;
+    NPBCG_HIP(hipMemcpyAsync(c->g.gh_learned, gh, sizeof(float) * 2 * c->n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    if (c->g.loaded == NPBCG_PRECOND_LEARNED) c->g.loaded = -1;
+    c->learned_ready = true;
+    return NPBCG_OK;
+}
+
 hipStream_t ctx_stream(const npbcg_ctx *c) { return c->stream; }
 int ctx_device(const npbcg_ctx *c) { return c->device; }
+int ctx_n(const npbcg_ctx *c) { return c->n; }
 
 }  // namespace npbcg_detail
 
 extern "C" {
 
 int npbcg_set_preconditioner(npbcg_ctx *c, int kind) {
-    if (!c || kind < NPBCG_PRECOND_NONE || kind > NPBCG_PRECOND_LEARNED)
+    if (!c || kind < NPBCG_PRECOND_NONE || kind > NPBCG_PRECOND_NEUMANN)
         return set_error(NPBCG_ERR_INVALID, "npbcg_set_preconditioner: invalid argument");
     if (kind == NPBCG_PRECOND_LEARNED && !c->learned_ready)
         return set_error(NPBCG_ERR_INVALID,
-                         "npbcg_set_preconditioner: no learned diagonal (npbcg_net_apply)");
+                         "npbcg_set_preconditioner: no learned preconditioner (npbcg_net_apply)");
     NPBCG_HIP(hipSetDevice(c->device));
     c->precond = kind;
     clear_error();
@@ -933,16 +1137,144 @@ int npbcg_get_preconditioner(npbcg_ctx *c, double *minv) {
     NPBCG_HIP(hipSetDevice(c->device));
     int rc = select_precond(c, c->precond);
     if (rc != NPBCG_OK) return rc;
-    if (!c->pbs.minv) {
-        for (int j = 0; j < c->n; j++) minv[j] = 1.0;
-    } else {
-        NPBCG_HIP(hipMemcpyAsync(c->h_stage, c->pbs.minv, sizeof(double) * c->n,
+    const size_t bytes = sizeof(double) * c->n;
+    if (c->pbs.minv) {
+        NPBCG_HIP(hipMemcpyAsync(c->h_stage, c->pbs.minv, bytes, hipMemcpyDeviceToHost,
+                                 c->stream));
+        NPBCG_HIP(hipStreamSynchronize(c->stream));
+        std::memcpy(minv, c->h_stage, bytes);
+    } else if (c->pbs.pop) {  // slot 0 of every row of the SELL-64 operator
+        const auto &g = c->g;
+        std::vector<double> val(g.h_off.back());
+        NPBCG_HIP(hipMemcpyAsync(val.data(), g.op_val, sizeof(double) * val.size(),
                                  hipMemcpyDeviceToHost, c->stream));
         NPBCG_HIP(hipStreamSynchronize(c->stream));
-        std::memcpy(minv, c->h_stage, sizeof(double) * c->n);
+        for (int j = 0; j < c->n; j++)
+            minv[j] = val[(size_t)g.h_off[j / npbcg_dev::kWave] + j % npbcg_dev::kWave];
+    } else {
+        for (int j = 0; j < c->n; j++) minv[j] = 1.0;
     }
     clear_error();
     return NPBCG_OK;
+}
+
+int npbcg_apply_preconditioner(npbcg_ctx *c, const double *v, double *z) {
+    if (!c || !v || !z) return set_error(NPBCG_ERR_INVALID, "npbcg_apply_preconditioner: NULL");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = select_precond(c, c->precond);
+    if (rc != NPBCG_OK) return rc;
+    const int n = c->n;
+    const size_t bytes = sizeof(double) * n;
+    double *dv = c->pbs.q, *dz = c->pbs.p;  // solve-view scratch (rebuilt by every solve)
+    std::memcpy(c->h_stage, v, bytes);
+    NPBCG_HIP(hipMemcpyAsync(dv, c->h_stage, bytes, hipMemcpyHostToDevice, c->stream));
+    if (c->pbs.pop) {
+        NPBCG_HIP(npbcg_dev::launch_precond_apply(*c->pbs.pop, dv, dz, nullptr, c->pbs.part.xx,
+                                                  c->pb.vec_blocks, nullptr, c->stream));
+        NPBCG_HIP(hipMemcpyAsync(c->h_stage, dz, bytes, hipMemcpyDeviceToHost, c->stream));
+        NPBCG_HIP(hipStreamSynchronize(c->stream));
+        std::memcpy(z, c->h_stage, bytes);
+    } else {
+        NPBCG_HIP(hipStreamSynchronize(c->stream));
+        std::vector<double> m(n);
+        rc = npbcg_get_preconditioner(c, m.data());
+        if (rc != NPBCG_OK) return rc;
+        for (int j = 0; j < n; j++) z[j] = m[j] * v[j];
+    }
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_log_pairs(npbcg_ctx *c, int pairs, float *r, float *e) {
+    if (!c || pairs < 1 || pairs > NPBCG_MAX_PAIRS)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_log_pairs: invalid argument");
+    if (c->rhs_norm <= 0.0)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_log_pairs: needs a nonzero right-hand side");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = ensure_pairs(c);
+    if (rc != NPBCG_OK) return rc;
+    // x* from plain FP64 CG
+    const int kind = c->precond;
+    c->precond = NPBCG_PRECOND_NONE;
+    npbcg_solve_stats st{};
+    rc = npbcg_solve(c, NPBCG_SOLVE_FP64, 1e-12, 5000, &st);
+    c->precond = kind;
+    if (rc != NPBCG_OK) return rc;
+    auto &pr = c->pairs;
+    const int n = c->n, vb = c->pb.vec_blocks;
+    NPBCG_HIP(hipMemcpyAsync(pr.xstar, c->pbs.z, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                             c->stream));
+    // the same CG from x = 0, logging (r_k, x* - x_k) before step k
+    Fp64Problem lp = c->pbs;
+    lp.minv = nullptr;
+    lp.pop = nullptr;
+    lp.pz = nullptr;
+    NPBCG_HIP(npbcg_dev::launch_reset_scalars(lp, 0.0, c->stream, 0.0, 0));
+    NPBCG_HIP(npbcg_dev::launch_cg_init(lp, 0, c->stream));
+    for (int k = 0; k < pairs; k++) {
+        NPBCG_HIP(npbcg_dev::launch_log_pair(lp.r, lp.z, pr.xstar, n, pr.r + (size_t)k * n,
+                                             pr.e + (size_t)k * n, pr.part + (size_t)k * vb, vb,
+                                             c->stream));
+        if (k + 1 < pairs) NPBCG_HIP(npbcg_dev::enqueue_solve_step(lp, k, 0, c->stream));
+    }
+    NPBCG_HIP(npbcg_dev::launch_pair_weights(pr.part, vb, pairs, pr.omega, c->stream));
+    NPBCG_HIP(hipMemcpyAsync(c->pbs.z, pr.xstar, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                             c->stream));  // npbcg_get_solution returns x*
+    pr.count = pairs;
+    if (r)
+        NPBCG_HIP(hipMemcpyAsync(r, pr.r, sizeof(float) * pairs * (size_t)n,
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (e)
+        NPBCG_HIP(hipMemcpyAsync(e, pr.e, sizeof(float) * pairs * (size_t)n,
+                                 hipMemcpyDeviceToHost, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_set_pairs(npbcg_ctx *c, int pairs, const float *r, const float *e) {
+    if (!c || !r || !e || pairs < 1 || pairs > NPBCG_MAX_PAIRS)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_set_pairs: invalid argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = ensure_pairs(c);
+    if (rc != NPBCG_OK) return rc;
+    const int n = c->n;
+    std::vector<float> omega(pairs);
+    for (int k = 0; k < pairs; k++) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += (double)e[(size_t)k * n + j] * e[(size_t)k * n + j];
+        omega[k] = s > 0.0 ? (float)(1.0 / (pairs * s)) : 0.0f;
+    }
+    auto &pr = c->pairs;
+    NPBCG_HIP(hipMemcpyAsync(pr.r, r, sizeof(float) * pairs * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream));
+    NPBCG_HIP(hipMemcpyAsync(pr.e, e, sizeof(float) * pairs * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream));
+    NPBCG_HIP(hipMemcpyAsync(pr.omega, omega.data(), sizeof(float) * pairs,
+                             hipMemcpyHostToDevice, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    pr.count = pairs;
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_graph(npbcg_ctx *c, int *nbr, float *wgt, float *nval, float *feat) {
+    if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_graph: NULL context");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = ensure_row_graph(c);
+    if (rc != NPBCG_OK) return rc;
+    const size_t nk = (size_t)c->n * npbcg_dev::kNbr;
+    const auto &g = c->g;
+    if (nbr) NPBCG_HIP(hipMemcpyAsync(nbr, g.nbr, sizeof(int) * nk, hipMemcpyDeviceToHost, c->stream));
+    if (wgt) NPBCG_HIP(hipMemcpyAsync(wgt, g.wgt, sizeof(float) * nk, hipMemcpyDeviceToHost, c->stream));
+    if (nval)
+        NPBCG_HIP(hipMemcpyAsync(nval, g.nval, sizeof(float) * nk, hipMemcpyDeviceToHost, c->stream));
+    if (feat)
+        NPBCG_HIP(hipMemcpyAsync(feat, g.feat, sizeof(float) * c->n * npbcg_dev::kFeat,
+                                 hipMemcpyDeviceToHost, c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    clear_error();
+    return npbcg_dev::kNbr;
 }
 
 int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats *st) {

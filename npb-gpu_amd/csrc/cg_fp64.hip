@@ -23,6 +23,7 @@
 
 #include "cg_device.h"
 #include "cg_kernels.h"
+#include "learned_kernels.h"
 
 namespace npbcg_dev {
 
@@ -597,7 +598,11 @@ hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
 hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st) {
     cg_init_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.p, pb.q, pb.r, pb.n,
                                                      scale_x, pb.sc, pb.part.xx, pb.minv);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !pb.pop) return e;
+    // sparse M^-1: p = z = M^-1 r, rho = r.z
+    return launch_precond_apply(*pb.pop, pb.r, pb.pz, pb.p, pb.part.xx, pb.vec_blocks, nullptr,
+                                st);
 }
 
 hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
@@ -614,8 +619,8 @@ hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
 // p = r + beta p closing CG step `step`; the result always lands in pb.p.
 hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st) {
     update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
-        pb.r, p_of_step(pb, step), pb.p, pb.n, pb.part.rr[step & 1], rho_source(pb, step),
-        pb.vec_blocks, pb.sc, pb.minv);
+        pb.pop ? pb.pz : pb.r, p_of_step(pb, step), pb.p, pb.n, pb.part.rr[step & 1],
+        rho_source(pb, step), pb.vec_blocks, pb.sc, pb.minv);
     return hipGetLastError();
 }
 
@@ -674,14 +679,21 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
     return e;
 }
 
+hipError_t enqueue_solve_step(const Fp64Problem &pb, int it, int sets_tol, hipStream_t st) {
+    hipError_t e = launch_spmv<0>(pb, st, nullptr, nullptr, it);
+    if (e == hipSuccess) e = launch_update_zr(pb, it, st, sets_tol);
+    if (e == hipSuccess && pb.pop)  // z = M^-1 r; its r.z replaces the r.r partials
+        e = launch_precond_apply(*pb.pop, pb.r, pb.pz, nullptr, pb.part.rr[it & 1],
+                                 pb.vec_blocks, pb.sc, st);
+    if (e == hipSuccess) e = launch_update_p(pb, it, st);
+    return e;
+}
+
 hipError_t enqueue_solve_steps(const Fp64Problem &pb, int steps, int sets_tol,
                                hipStream_t st) {
     hipError_t e = hipSuccess;
-    for (int it = 0; it < steps && e == hipSuccess; it++) {
-        e = launch_spmv<0>(pb, st, nullptr, nullptr, it);
-        if (e == hipSuccess) e = launch_update_zr(pb, it, st, sets_tol);
-        if (e == hipSuccess) e = launch_update_p(pb, it, st);
-    }
+    for (int it = 0; it < steps && e == hipSuccess; it++)
+        e = enqueue_solve_step(pb, it, sets_tol, st);
     return e;
 }
 

@@ -44,6 +44,8 @@ struct Partials {
     double *nrm;     // x.z, z.z of norms_kernel  [2 * vec_blocks]
 };
 
+struct PrecondOp;  // sparse preconditioner operator (learned_kernels.h)
+
 // Device pointers and launch shapes of one FP64 problem.
 struct Fp64Problem {
     int n;
@@ -62,6 +64,8 @@ struct Fp64Problem {
     int spmv_variant;
     int fuse_p_update;          // SELL path: p update folded into the next SpMV
     const double *minv;         // diagonal preconditioner M^-1 (nullptr: plain CG)
+    const PrecondOp *pop;       // sparse preconditioner (nullptr: none); z = M^-1 r in pz
+    double *pz;
     SellDev sell;
 };
 
@@ -116,6 +120,8 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
 
 // `steps` CG steps (SpMV, update_zr, update_p) of a solve on pb (x holds b,
 // z the iterate); the stop flag makes steps after convergence return at once.
+// Step `it` of a solve sequence (it selects the partial-sum buffers).
+hipError_t enqueue_solve_step(const Fp64Problem &pb, int it, int sets_tol, hipStream_t st);
 hipError_t enqueue_solve_steps(const Fp64Problem &pb, int steps, int sets_tol,
                                hipStream_t st);
 
@@ -123,6 +129,8 @@ hipError_t enqueue_solve_steps(const Fp64Problem &pb, int steps, int sets_tol,
 struct MixedProblem {
     const float *val = nullptr;   // FP32 values in the SELL order of the FP64 copy
     const float *minv = nullptr;  // FP32 diagonal preconditioner (nullptr: none)
+    const PrecondOp *pop = nullptr;  // sparse preconditioner (FP32 values); z in pz
+    float *pz = nullptr;
     float *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;  // n+2 each
     float *rows_part = nullptr;   // nblk * n per-block row sums
     double *part_xx = nullptr;    // r0.r0 partials of an inner start [vec_blocks]

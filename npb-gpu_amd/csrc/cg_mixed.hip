@@ -12,6 +12,7 @@
 // structure and the partial-sum handoff are those of the FP64 path.
 #include "cg_device.h"
 #include "cg_kernels.h"
+#include "learned_kernels.h"
 
 namespace npbcg_dev {
 
@@ -241,6 +242,8 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
     init32_kernel<<<vb, kBlock, 0, st>>>(pb.x, pb.r, first_sweep, pb.n, mp.d, mp.r, mp.p,
                                          mp.part_xx, mp.minv);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess && mp.pop)  // p = z = M^-1 r, rho = r.z
+        e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, mp.p, mp.part_xx, vb, nullptr, st);
     for (int it = 0; it < steps && e == hipSuccess; it++) {
         const SellDev &sd = pb.sell;
         if (sd.block == 512)
@@ -259,8 +262,14 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
                                                   mp.sc, it == 0 ? 1 : 0, mp.minv);
         e = hipGetLastError();
         if (e != hipSuccess) break;
-        update_p32_kernel<<<vb, kBlock, 0, st>>>(mp.r, mp.p, pb.n, mp.part_rr[it & 1],
-                                                 rho32_source(mp, it), mp.sc, mp.minv);
+        if (mp.pop) {
+            e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, nullptr, mp.part_rr[it & 1], vb,
+                                       mp.sc, st);
+            if (e != hipSuccess) break;
+        }
+        update_p32_kernel<<<vb, kBlock, 0, st>>>(mp.pop ? mp.pz : mp.r, mp.p, pb.n,
+                                                 mp.part_rr[it & 1], rho32_source(mp, it), mp.sc,
+                                                 mp.minv);
         e = hipGetLastError();
     }
     return e;
@@ -274,7 +283,10 @@ hipError_t launch_refine(const Fp64Problem &pb, const MixedProblem &mp, hipStrea
 hipError_t launch_residual_restart(const Fp64Problem &pb, hipStream_t st) {
     residual_restart_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.r, pb.p, pb.n,
                                                               pb.part.xx, pb.minv);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !pb.pop) return e;
+    return launch_precond_apply(*pb.pop, pb.r, pb.pz, pb.p, pb.part.xx, pb.vec_blocks, nullptr,
+                                st);
 }
 
 hipError_t launch_narrow(const double *in, float *out, int n, int blocks, hipStream_t st) {

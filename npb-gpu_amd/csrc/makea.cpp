@@ -41,7 +41,11 @@ struct Lcg46 {
     }
 };
 
+thread_local int t_gen_threads = 0;  // per-thread override (0: none)
+
 int gen_threads() {
+    if (t_gen_threads > 0)
+        return t_gen_threads;
     const char *env = std::getenv("NPBCG_GEN_THREADS");
     if (env && std::atoi(env) > 0)
         return std::atoi(env);
@@ -70,6 +74,13 @@ void parallel_for(int nthreads, long count, F &&body) {
 }
 
 }  // namespace
+
+namespace npbcg_detail {
+
+// Worker threads of the generator used by npbcg_makea calls from this thread.
+void set_generator_threads(int threads) { t_gen_threads = threads; }
+
+}  // namespace npbcg_detail
 
 extern "C" long npbcg_makea(int n, int nonzer, double rcond, double shift,
                             double seed, long nz_cap, int *rowstr, int *colidx,

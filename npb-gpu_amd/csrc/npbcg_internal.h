@@ -10,6 +10,10 @@ namespace npbcg_detail {
 int set_error(int code, const std::string &msg);
 void clear_error();
 
+// Worker threads of the generator used by npbcg_makea calls from the calling
+// thread (0: the default, NPBCG_GEN_THREADS or up to 16).
+void set_generator_threads(int threads);
+
 }  // namespace npbcg_detail
 
 #endif

@@ -33,6 +33,7 @@ SPMV_AUTO, SPMV_SEG64, SPMV_SEG32, SPMV_SEG16, SPMV_SEG64_U2, SPMV_TILED = 0, 1,
 
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _DP = ctypes.POINTER(ctypes.c_double)
 
 
@@ -51,7 +52,30 @@ class Opts(ctypes.Structure):
 
 
 SOLVE_FP64, SOLVE_MIXED = 0, 1
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_LEARNED = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_NEUMANN = 0, 1, 2, 3
+MAX_PAIRS = 8
+NET_FEATURES = 8
+
+
+class TrainOpts(ctypes.Structure):
+    _fields_ = [("systems", ctypes.c_int), ("n_min", ctypes.c_int), ("n_max", ctypes.c_int),
+                ("nonzer_min", ctypes.c_int), ("nonzer_max", ctypes.c_int),
+                ("shift_min", ctypes.c_double), ("shift_max", ctypes.c_double),
+                ("rcond", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+                ("pairs", ctypes.c_int), ("steps_per_system", ctypes.c_int),
+                ("lr", ctypes.c_float), ("gen_threads", ctypes.c_int),
+                ("first_system", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+
+
+class TrainStats(ctypes.Structure):
+    _fields_ = [("systems", ctypes.c_int), ("steps", ctypes.c_int),
+                ("loss_first", ctypes.c_double), ("loss_last", ctypes.c_double),
+                ("base_first", ctypes.c_double), ("base_last", ctypes.c_double),
+                ("wait_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("reserved", ctypes.c_double * 4)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class SolveStats(ctypes.Structure):
@@ -111,6 +135,22 @@ def _load() -> ctypes.CDLL:
         "npbcg_get_solution": (ctypes.c_int, [vp, _F64P]),
         "npbcg_set_preconditioner": (ctypes.c_int, [vp, ctypes.c_int]),
         "npbcg_get_preconditioner": (ctypes.c_int, [vp, _F64P]),
+        "npbcg_apply_preconditioner": (ctypes.c_int, [vp, _F64P, _F64P]),
+        "npbcg_log_pairs": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
+        "npbcg_set_pairs": (ctypes.c_int, [vp, ctypes.c_int, _F32P, _F32P]),
+        "npbcg_graph": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "npbcg_net_param_count": (ctypes.c_int, []),
+        "npbcg_net_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int,
+                                            ctypes.c_ulonglong]),
+        "npbcg_net_destroy": (None, [vp]),
+        "npbcg_net_get_params": (ctypes.c_int, [vp, vp, vp]),
+        "npbcg_net_set_params": (ctypes.c_int, [vp, vp, vp]),
+        "npbcg_net_default_train_opts": (None, [ctypes.POINTER(TrainOpts)]),
+        "npbcg_net_train": (ctypes.c_int, [vp, ctypes.POINTER(TrainOpts),
+                                           ctypes.POINTER(TrainStats), vp, vp]),
+        "npbcg_net_loss_grad": (ctypes.c_int, [vp, vp, _DP, vp, vp]),
+        "npbcg_net_step": (ctypes.c_int, [vp, vp, ctypes.c_float, _DP]),
+        "npbcg_net_apply": (ctypes.c_int, [vp, vp, _DP]),
         "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
         "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
                                           _F64P, _F64P, _DP]),
@@ -295,10 +335,42 @@ class Context:
         _check(LIB.npbcg_set_preconditioner(self._h, kind), "npbcg_set_preconditioner")
 
     def preconditioner(self) -> np.ndarray:
-        """The active M^-1 diagonal (ones for plain CG)."""
+        """The diagonal of the active M^-1 (ones for plain CG)."""
         m = np.zeros(self.n)
         _check(LIB.npbcg_get_preconditioner(self._h, m), "npbcg_get_preconditioner")
         return m
+
+    def apply_preconditioner(self, v: np.ndarray) -> np.ndarray:
+        z = np.zeros(self.n)
+        _check(LIB.npbcg_apply_preconditioner(
+            self._h, np.ascontiguousarray(v, dtype=np.float64), z),
+            "npbcg_apply_preconditioner")
+        return z
+
+    def log_pairs(self, pairs: int = MAX_PAIRS):
+        """CG residual/correction pairs of the resident right-hand side."""
+        r = np.zeros((pairs, self.n), np.float32)
+        e = np.zeros((pairs, self.n), np.float32)
+        _check(LIB.npbcg_log_pairs(self._h, pairs, r.ctypes.data, e.ctypes.data),
+               "npbcg_log_pairs")
+        return r, e
+
+    def set_pairs(self, r: np.ndarray, e: np.ndarray):
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        _check(LIB.npbcg_set_pairs(self._h, r.shape[0], r, e), "npbcg_set_pairs")
+
+    def graph(self):
+        """(nbr, wgt, nval, feat) of the top-K row graph."""
+        k = 4
+        nbr = np.zeros((self.n, k), np.int32)
+        wgt = np.zeros((self.n, k), np.float32)
+        nval = np.zeros((self.n, k), np.float32)
+        feat = np.zeros((self.n, NET_FEATURES), np.float32)
+        kk = _check(LIB.npbcg_graph(self._h, nbr.ctypes.data, wgt.ctypes.data,
+                                    nval.ctypes.data, feat.ctypes.data), "npbcg_graph")
+        assert kk == k
+        return nbr, wgt, nval, feat
 
     def copy_bench(self, nbytes: int, reps: int = 10) -> float:
         """Mean ms of a device copy of nbytes (read nbytes + write nbytes)."""
@@ -310,4 +382,86 @@ class Context:
     def spmv_bench(self, reps: int = 20) -> float:
         ms = ctypes.c_double(0.0)
         _check(LIB.npbcg_spmv_bench(self._h, reps, ctypes.byref(ms)), "npbcg_spmv_bench")
+        return ms.value
+
+
+def default_train_opts() -> TrainOpts:
+    o = TrainOpts()
+    LIB.npbcg_net_default_train_opts(ctypes.byref(o))
+    return o
+
+
+class Net:
+    """The learned-preconditioner network (npbcg_net_*)."""
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self._h = ctypes.c_void_p()
+        _check(LIB.npbcg_net_create(ctypes.byref(self._h), device, seed), "npbcg_net_create")
+        self.param_count = LIB.npbcg_net_param_count()
+
+    def close(self):
+        if self._h:
+            LIB.npbcg_net_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.param_count, np.float32)
+        nrm = np.zeros(2 * NET_FEATURES, np.float32)
+        _check(LIB.npbcg_net_get_params(self._h, p.ctypes.data, nrm.ctypes.data),
+               "npbcg_net_get_params")
+        return p, nrm
+
+    def set_params(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        nrm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        _check(LIB.npbcg_net_set_params(self._h, None if p is None else p.ctypes.data,
+                                        None if nrm is None else nrm.ctypes.data),
+               "npbcg_net_set_params")
+
+    def save(self, path: str):
+        p, nrm = self.get_params()
+        np.savez(path, params=p, norm=nrm)
+
+    def load(self, path: str):
+        with np.load(path, allow_pickle=False) as f:
+            self.set_params(f["params"], f["norm"])
+
+    def train(self, opts: Optional[TrainOpts] = None, history: bool = True):
+        o = opts if opts is not None else default_train_opts()
+        st = TrainStats()
+        lh = np.zeros(o.systems) if history else None
+        bh = np.zeros(o.systems) if history else None
+        _check(LIB.npbcg_net_train(self._h, ctypes.byref(o), ctypes.byref(st),
+                                   None if lh is None else lh.ctypes.data,
+                                   None if bh is None else bh.ctypes.data), "npbcg_net_train")
+        return st, lh, bh
+
+    def loss_grad(self, ctx: "Context", n: int):
+        loss = ctypes.c_double(0.0)
+        grad = np.zeros(self.param_count, np.float32)
+        gh = np.zeros(2 * n, np.float32)
+        _check(LIB.npbcg_net_loss_grad(self._h, ctx._h, ctypes.byref(loss), grad.ctypes.data,
+                                       gh.ctypes.data), "npbcg_net_loss_grad")
+        return loss.value, grad, gh
+
+    def step(self, ctx: "Context", lr: float = 1e-3) -> float:
+        loss = ctypes.c_double(0.0)
+        _check(LIB.npbcg_net_step(self._h, ctx._h, lr, ctypes.byref(loss)), "npbcg_net_step")
+        return loss.value
+
+    def apply(self, ctx: "Context") -> float:
+        ms = ctypes.c_double(0.0)
+        _check(LIB.npbcg_net_apply(self._h, ctx._h, ctypes.byref(ms)), "npbcg_net_apply")
         return ms.value
