@@ -304,6 +304,64 @@ int npbcg_net_step(npbcg_net *net, npbcg_ctx *ctx, float lr, double *loss);
  * preconditioner; ms (may be NULL) receives the device time. */
 int npbcg_net_apply(npbcg_net *net, npbcg_ctx *ctx, double *ms);
 
+/* ---- batched solves ----------------------------------------------------------
+ * Many independent systems, each with nrhs right-hand sides, solved together:
+ * the matrices are concatenated block-diagonally on the device and one CG step
+ * of every unfinished system is one launch sequence (SpMM over all systems with
+ * the nrhs vectors read together, x/r update, p update).  MIXED runs the inner
+ * CG in FP32 inside FP64 refinement sweeps (FP64 true residual per sweep);
+ * FP64 runs the same kernels in FP64 with one sweep.  There is no counterpart
+ * in the reference, which solves one system per process. */
+typedef struct npbcg_batch npbcg_batch;
+
+typedef struct npbcg_batch_opts {
+    int device;
+    int nrhs;            /* 1, 2 or 4 */
+    int mode;            /* NPBCG_SOLVE_FP64 or NPBCG_SOLVE_MIXED */
+    int lanes_per_row;   /* SpMM lanes per row: 4, 8, 16, 32, 64; 0 = tuned or default */
+    int rows_per_tile;   /* rows per workgroup tile: 64 .. 4096; 0 = tuned or default */
+    int inner_steps;     /* MIXED: inner CG steps per sweep at most (0: 60) */
+    int reserved[10];
+} npbcg_batch_opts;
+
+typedef struct npbcg_batch_stats {
+    int solves;          /* systems * nrhs */
+    int converged;       /* solves whose FP64 true relative residual <= rtol */
+    int sweeps;          /* FP64 residual checks */
+    int max_steps;       /* largest inner CG step count of a solve */
+    int lanes_per_row;
+    int rows_per_tile;
+    int reserved_i[2];
+    double time_ms;      /* HIP events around the device work */
+    double max_relres;
+    double reserved_d[4];
+} npbcg_batch_stats;
+
+void npbcg_batch_default_opts(npbcg_batch_opts *opts);
+/* Uploads nsys CSR matrices (0-based, n[s] rows each). */
+int npbcg_batch_create(npbcg_batch **batch, int nsys, const int *n, const int *const *rowstr,
+                       const int *const *colidx, const double *const *a,
+                       const npbcg_batch_opts *opts);
+void npbcg_batch_destroy(npbcg_batch *batch);
+/* Right-hand sides: system after system, each n[s] rows of nrhs values. */
+int npbcg_batch_set_rhs(npbcg_batch *batch, const double *b);
+/* Solves every system to rtol (FP64 true residual); maxit bounds the CG steps
+ * of each solve.  steps and relres (nsys*nrhs each, may be NULL) receive the
+ * per-solve results.  NPBCG_ERR_NOCONV if a solve missed rtol. */
+int npbcg_batch_solve(npbcg_batch *batch, double rtol, int maxit, npbcg_batch_stats *stats,
+                      int *steps, double *relres);
+int npbcg_batch_get_solution(npbcg_batch *batch, double *x);
+/* Autotuner: times the SpMM of this batch for every (lanes_per_row,
+ * rows_per_tile) candidate, keeps the fastest, and records it in the text
+ * file cache_path (may be NULL) under the batch's shape; a later call with the
+ * same shape and file reuses the record without timing.  ms (may be NULL)
+ * receives the chosen SpMM time. */
+int npbcg_batch_tune(npbcg_batch *batch, const char *cache_path, int *lanes_per_row,
+                     int *rows_per_tile, double *ms);
+/* Generator threads of npbcg_makea calls made from the calling thread
+ * (0: the default, NPBCG_GEN_THREADS or up to 16). */
+void npbcg_set_generator_threads(int threads);
+
 /* ---- reference-shaped shim ------------------------------------------------
  * For a caller that keeps the reference's exact call
  *   conj_grad(colidx, rowstr, x, z, a, p, q, r, &rnorm);

@@ -1,0 +1,65 @@
+// Batched CG over independent systems with NR right-hand sides each
+// (batch_kernels.hip): device layout and launchers.
+#ifndef NPBCG_BATCH_KERNELS_H
+#define NPBCG_BATCH_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+namespace npbcg_dev {
+
+// Per (system, right-hand side) scalars of the inner CG.
+struct BatchScalars {
+    double rtol2;     // inner stop level relative to the start r.r (set by the host)
+    double tol2;      // rtol2 * r0.r0, set by the first step
+    double rho;       // r.r of the last step
+    int steps;        // inner CG steps, all sweeps
+    int sweep_steps;  // inner CG steps of this sweep
+};
+
+// The concatenated batch (device pointers) and its launch shape.
+struct BatchDev {
+    int nsys = 0;
+    int nr = 1;          // right-hand sides per system (1, 2 or 4)
+    int lpr = 16;        // lanes per row of the SpMM (4 .. 64)
+    int tiles = 0;       // workgroups; tiles never cross systems
+    int rows = 0;        // total rows
+    int maxit = 0;       // inner steps per sweep
+    const int *tile_sys = nullptr;  // [tiles]
+    const int *tile_row = nullptr;  // [tiles + 1] first row of each tile
+    const int *sys_tile = nullptr;  // [nsys + 1] first tile of each system
+    const int *rowptr = nullptr;    // [rows + 1]
+    const int *col = nullptr;       // global column indices
+    int *done = nullptr;            // [nsys * nr] right-hand side finished
+    int *done_next = nullptr;       // flags set during a step, published after it
+    BatchScalars *sc = nullptr;     // [nsys * nr]
+};
+
+// Inner-solve vectors ([rows][nr]) and partial sums ([tiles][nr]) in type T.
+template <typename T>
+struct BatchVecs {
+    const T *val = nullptr;   // matrix values in type T
+    T *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    double *part_init = nullptr, *part_pq = nullptr, *part_rr[2] = {nullptr, nullptr};
+};
+
+// Step `step` of an inner solve of every unfinished system (3 launches + publish).
+template <typename T>
+hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, int first,
+                         hipStream_t st);
+// q = A p and p.q partials only (the SpMM the autotuner times).
+template <typename T>
+hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st);
+// r = p = (T) res, d = 0 for the unfinished right-hand sides.
+template <typename T>
+hipError_t batch_init(const BatchDev &bd, const BatchVecs<T> &v, const double *res,
+                      hipStream_t st);
+template <typename T>
+hipError_t batch_refine(const BatchDev &bd, double *x, const T *d, hipStream_t st);
+// res = b - A x (FP64) and |res|^2 per (system, rhs) into sys_out[nsys * nr].
+hipError_t batch_residual(const BatchDev &bd, const double *val, const double *b, const double *x,
+                          double *res, double *part, double *sys_out, hipStream_t st);
+hipError_t batch_narrow(const double *in, float *out, long long count, hipStream_t st);
+
+}  // namespace npbcg_dev
+
+#endif

@@ -1,0 +1,488 @@
+// Batched CG over many independent systems with several right-hand sides
+// each (gfx950).
+//
+// All systems are concatenated block-diagonally: one CSR with global row and
+// column indices, vectors stored row-major with the NR right-hand sides of a
+// row next to each other ([row][NR]).  Work is cut into tiles of consecutive
+// rows that never cross a system boundary, one workgroup per tile, so every
+// per-system reduction is a fixed-order sum over that system's tile partials
+// and no atomics are needed.  One CG step for the whole batch is three
+// launches (SpMM + p.q, x/r update + r.r, p update + stop test); a system whose
+// right-hand sides have all converged costs its tiles one flag read.
+//
+// The SpMM reads each matrix entry once for all NR vectors: LPR lanes share a
+// row, every lane gathers the NR values of its column as one vector load.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "batch_kernels.h"
+#include "cg_device.h"
+
+namespace npbcg_dev {
+
+namespace {
+
+template <typename T, int NR>
+struct VecT {
+    T v[NR];
+};
+
+template <typename T, int NR>
+__device__ __forceinline__ VecT<T, NR> load_vec(const T *__restrict__ p) {
+    VecT<T, NR> r;
+    if constexpr (sizeof(T) * NR == 16) {
+        typedef __attribute__((__vector_size__(16))) T v16;
+        const v16 x = *reinterpret_cast<const v16 *>(p);
+#pragma unroll
+        for (int k = 0; k < NR; k++) r.v[k] = x[k];
+    } else if constexpr (sizeof(T) * NR == 32) {
+        typedef __attribute__((__vector_size__(16))) T v16;
+        const v16 x0 = *reinterpret_cast<const v16 *>(p);
+        const v16 x1 = *reinterpret_cast<const v16 *>(p + NR / 2);
+#pragma unroll
+        for (int k = 0; k < NR / 2; k++) {
+            r.v[k] = x0[k];
+            r.v[NR / 2 + k] = x1[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NR; k++) r.v[k] = p[k];
+    }
+    return r;
+}
+
+// Block sums of NR per-thread doubles; thread 0 writes out[0..NR).
+template <int NR>
+__device__ __forceinline__ void block_sum_store(const double (&acc)[NR], double *out) {
+    __shared__ double ws[NR][kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const double s = segment_sum<64>(acc[k]);
+        if (lane == 0) ws[k][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            double t = ws[k][0];
+            for (int w = 1; w < kWavesPerBlock; w++) t += ws[k][w];
+            out[k] = t;
+        }
+    }
+}
+
+// out[k] = sum over tiles t in [t0, t1) of part[t * NR + k], in a fixed order
+// (every workgroup of the system gets the same bits).  All threads call it.
+template <int NR>
+__device__ __forceinline__ void system_sum(const double *__restrict__ part, int t0, int t1,
+                                           double (&out)[NR]) {
+    __shared__ double ws[NR][kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    for (int t = t0 + threadIdx.x; t < t1; t += kBlock)
+#pragma unroll
+        for (int k = 0; k < NR; k++) acc[k] += part[(size_t)t * NR + k];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const double s = segment_sum<64>(acc[k]);
+        if (lane == 0) ws[k][wid] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        double t = ws[k][0];
+        for (int w = 1; w < kWavesPerBlock; w++) t += ws[k][w];
+        out[k] = t;
+    }
+    __syncthreads();
+}
+
+template <int NR>
+__device__ __forceinline__ bool system_done(const int *__restrict__ done, int s) {
+    bool all = true;
+#pragma unroll
+    for (int k = 0; k < NR; k++) all = all && done[s * NR + k] != 0;
+    return all;
+}
+
+}  // namespace
+
+// q = A p over one tile (LPR lanes per row), q stored, partials of p.q per RHS.
+template <typename T, int NR, int LPR>
+__global__ __launch_bounds__(kBlock) void batch_spmm_kernel(BatchDev bd, const T *__restrict__ val,
+                                                            const T *__restrict__ p,
+                                                            T *__restrict__ q,
+                                                            double *__restrict__ pq_part) {
+    const int t = blockIdx.x;
+    const int s = bd.tile_sys[t];
+    if (system_done<NR>(bd.done, s)) return;
+    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
+    const int grp = threadIdx.x / LPR, sub = threadIdx.x % LPR;
+    constexpr int kGroups = kBlock / LPR;
+    double dot[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) dot[k] = 0.0;
+    for (int row = row0 + grp; row < row1; row += kGroups) {
+        T acc[NR];
+#pragma unroll
+        for (int k = 0; k < NR; k++) acc[k] = T(0);
+        const int e1 = bd.rowptr[row + 1];
+        for (int e = bd.rowptr[row] + sub; e < e1; e += LPR) {
+            const T a = val[e];
+            const VecT<T, NR> x = load_vec<T, NR>(p + (size_t)bd.col[e] * NR);
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] = fma(a, x.v[k], acc[k]);
+        }
+#pragma unroll
+        for (int m = LPR / 2; m >= 1; m >>= 1)
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] += __shfl_xor(acc[k], m, kWave);
+        if (sub == 0) {
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                q[(size_t)row * NR + k] = acc[k];
+                dot[k] += (double)p[(size_t)row * NR + k] * (double)acc[k];
+            }
+        }
+    }
+    block_sum_store<NR>(dot, pq_part + (size_t)t * NR);
+}
+
+// alpha = rho / p.q per RHS of the tile's system; d += alpha p, r -= alpha q;
+// partials of the new r.r.  The first step of a solve sets tol2 = rtol2 * rho.
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void batch_update_kernel(BatchDev bd, const T *__restrict__ p,
+                                                              const T *__restrict__ q,
+                                                              T *__restrict__ d, T *__restrict__ r,
+                                                              const double *__restrict__ rho_part,
+                                                              const double *__restrict__ pq_part,
+                                                              double *__restrict__ rr_part,
+                                                              int first_step) {
+    const int t = blockIdx.x;
+    const int s = bd.tile_sys[t];
+    if (system_done<NR>(bd.done, s)) return;
+    const int t0 = bd.sys_tile[s], t1 = bd.sys_tile[s + 1];
+    double rho[NR], pq[NR];
+    system_sum<NR>(rho_part, t0, t1, rho);
+    system_sum<NR>(pq_part, t0, t1, pq);
+    T alpha[NR];
+    bool act[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        act[k] = bd.done[s * NR + k] == 0;
+        alpha[k] = act[k] && pq[k] != 0.0 ? (T)(rho[k] / pq[k]) : T(0);
+    }
+    if (t == t0)
+#pragma unroll
+        for (int k = 0; k < NR; k++)
+            if (threadIdx.x == k) {
+                BatchScalars &sc = bd.sc[s * NR + k];
+                if (first_step) sc.tol2 = sc.rtol2 * fabs(rho[k]);
+                sc.rho = rho[k];
+                if (act[k]) {
+                    sc.steps += 1;
+                    sc.sweep_steps += 1;
+                }
+            }
+    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    for (int row = row0 + threadIdx.x; row < row1; row += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const size_t at = (size_t)row * NR + k;
+            d[at] = fma(alpha[k], p[at], d[at]);
+            const T rv = fma(-alpha[k], q[at], r[at]);
+            r[at] = rv;
+            acc[k] += (double)rv * (double)rv;
+        }
+    }
+    block_sum_store<NR>(acc, rr_part + (size_t)t * NR);
+}
+
+// beta = r.r / rho per RHS; stop test r.r <= tol2 (the tile's system's first
+// tile records it); p = r + beta p for the right-hand sides still running.
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void batch_pupdate_kernel(BatchDev bd,
+                                                               const T *__restrict__ r,
+                                                               T *__restrict__ p,
+                                                               const double *__restrict__ rr_part,
+                                                               const double *__restrict__ rho_part) {
+    const int t = blockIdx.x;
+    const int s = bd.tile_sys[t];
+    if (system_done<NR>(bd.done, s)) return;
+    const int t0 = bd.sys_tile[s], t1 = bd.sys_tile[s + 1];
+    double rr[NR], rho[NR];
+    system_sum<NR>(rr_part, t0, t1, rr);
+    system_sum<NR>(rho_part, t0, t1, rho);
+    T beta[NR];
+    bool run[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const BatchScalars &sc = bd.sc[s * NR + k];
+        const bool act = bd.done[s * NR + k] == 0;
+        const bool stop = act && (fabs(rr[k]) <= sc.tol2 || sc.sweep_steps >= bd.maxit);
+        run[k] = act && !stop;
+        beta[k] = run[k] && rho[k] != 0.0 ? (T)(rr[k] / rho[k]) : T(0);
+    }
+    if (t == t0)
+#pragma unroll
+        for (int k = 0; k < NR; k++)
+            if (threadIdx.x == k && bd.done[s * NR + k] == 0 && !run[k]) bd.done_next[s * NR + k] = 1;
+    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
+    for (int row = row0 + threadIdx.x; row < row1; row += kBlock)
+#pragma unroll
+        for (int k = 0; k < NR; k++)
+            if (run[k]) {
+                const size_t at = (size_t)row * NR + k;
+                p[at] = fma(beta[k], p[at], r[at]);
+            }
+}
+
+// done = done_next (the stop flags of the step just finished become visible
+// to the next step's kernels, never to the current step's other workgroups).
+__global__ void batch_publish_kernel(int *__restrict__ done, const int *__restrict__ done_next,
+                                     int count) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock)
+        done[i] = done_next[i];
+}
+
+// Start of an inner solve: r = p = (T)(b - A x) (given in res), d = 0,
+// partials of r.r.  Right-hand sides already done keep r = p = 0.
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void batch_init_kernel(BatchDev bd,
+                                                            const double *__restrict__ res,
+                                                            T *__restrict__ d, T *__restrict__ r,
+                                                            T *__restrict__ p,
+                                                            double *__restrict__ rr_part) {
+    const int t = blockIdx.x;
+    const int s = bd.tile_sys[t];
+    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
+    bool act[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) act[k] = bd.done[s * NR + k] == 0;
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    for (int row = row0 + threadIdx.x; row < row1; row += kBlock)
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const size_t at = (size_t)row * NR + k;
+            const T v = act[k] ? (T)res[at] : T(0);
+            d[at] = T(0);
+            r[at] = v;
+            p[at] = v;
+            acc[k] += (double)v * (double)v;
+        }
+    block_sum_store<NR>(acc, rr_part + (size_t)t * NR);
+}
+
+// res = b - A x in FP64 (LPR lanes per row), partials of |res|^2.
+template <int NR, int LPR>
+__global__ __launch_bounds__(kBlock) void batch_residual_kernel(BatchDev bd,
+                                                                const double *__restrict__ val,
+                                                                const double *__restrict__ b,
+                                                                const double *__restrict__ x,
+                                                                double *__restrict__ res,
+                                                                double *__restrict__ part) {
+    const int t = blockIdx.x;
+    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
+    const int grp = threadIdx.x / LPR, sub = threadIdx.x % LPR;
+    constexpr int kGroups = kBlock / LPR;
+    double nrm[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) nrm[k] = 0.0;
+    for (int row = row0 + grp; row < row1; row += kGroups) {
+        double acc[NR];
+#pragma unroll
+        for (int k = 0; k < NR; k++) acc[k] = 0.0;
+        const int e1 = bd.rowptr[row + 1];
+        for (int e = bd.rowptr[row] + sub; e < e1; e += LPR) {
+            const double a = val[e];
+            const VecT<double, NR> xv = load_vec<double, NR>(x + (size_t)bd.col[e] * NR);
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] = fma(a, xv.v[k], acc[k]);
+        }
+#pragma unroll
+        for (int m = LPR / 2; m >= 1; m >>= 1)
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] += __shfl_xor(acc[k], m, kWave);
+        if (sub == 0)
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                const size_t at = (size_t)row * NR + k;
+                const double rv = b[at] - acc[k];
+                res[at] = rv;
+                nrm[k] += rv * rv;
+            }
+    }
+    block_sum_store<NR>(nrm, part + (size_t)t * NR);
+}
+
+// One workgroup per system: out[s*NR + k] = sum of its tiles' partials.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void batch_system_sum_kernel(BatchDev bd,
+                                                                  const double *__restrict__ part,
+                                                                  double *__restrict__ out) {
+    const int s = blockIdx.x;
+    double v[NR];
+    system_sum<NR>(part, bd.sys_tile[s], bd.sys_tile[s + 1], v);
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+        if (threadIdx.x == k) out[s * NR + k] = v[k];
+}
+
+// x += (double) d.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void batch_refine_kernel(double *__restrict__ x,
+                                                              const T *__restrict__ d,
+                                                              long long count) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < count;
+         i += (long long)gridDim.x * kBlock)
+        x[i] += (double)d[i];
+}
+
+__global__ __launch_bounds__(kBlock) void batch_narrow_kernel(const double *__restrict__ in,
+                                                              float *__restrict__ out,
+                                                              long long count) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < count;
+         i += (long long)gridDim.x * kBlock)
+        out[i] = (float)in[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers (templates dispatched on NR in {1, 2, 4} and LPR in {4..64})
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int grid_for(long long items) {
+    return (int)std::max<long long>(1, std::min<long long>((items + kBlock - 1) / kBlock, 8192));
+}
+
+template <typename F>
+hipError_t with_lpr(int lpr, F &&f) {
+    switch (lpr) {
+        case 4: return f(std::integral_constant<int, 4>{});
+        case 8: return f(std::integral_constant<int, 8>{});
+        case 16: return f(std::integral_constant<int, 16>{});
+        case 32: return f(std::integral_constant<int, 32>{});
+        case 64: return f(std::integral_constant<int, 64>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename F>
+hipError_t with_nr(int nr, F &&f) {
+    switch (nr) {
+        case 1: return f(std::integral_constant<int, 1>{});
+        case 2: return f(std::integral_constant<int, 2>{});
+        case 4: return f(std::integral_constant<int, 4>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, int first,
+                         hipStream_t st) {
+    const int tiles = bd.tiles;
+    const double *rho = step == 0 ? v.part_init : v.part_rr[(step - 1) & 1];
+    double *rr = v.part_rr[step & 1];
+    hipError_t e = with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        return with_lpr(bd.lpr, [&](auto lc) {
+            constexpr int LPR = decltype(lc)::value;
+            batch_spmm_kernel<T, NR, LPR><<<tiles, kBlock, 0, st>>>(bd, v.val, v.p, v.q, v.part_pq);
+            return hipGetLastError();
+        });
+    });
+    if (e != hipSuccess) return e;
+    e = with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        batch_update_kernel<T, NR><<<tiles, kBlock, 0, st>>>(bd, v.p, v.q, v.d, v.r, rho,
+                                                             v.part_pq, rr, first);
+        hipError_t e2 = hipGetLastError();
+        if (e2 != hipSuccess) return e2;
+        batch_pupdate_kernel<T, NR><<<tiles, kBlock, 0, st>>>(bd, v.r, v.p, rr, rho);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return e;
+    batch_publish_kernel<<<grid_for(bd.nsys * bd.nr), kBlock, 0, st>>>(bd.done, bd.done_next,
+                                                                       bd.nsys * bd.nr);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st) {
+    return with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        return with_lpr(bd.lpr, [&](auto lc) {
+            constexpr int LPR = decltype(lc)::value;
+            batch_spmm_kernel<T, NR, LPR><<<bd.tiles, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
+                                                                       v.part_pq);
+            return hipGetLastError();
+        });
+    });
+}
+
+template <typename T>
+hipError_t batch_init(const BatchDev &bd, const BatchVecs<T> &v, const double *res,
+                      hipStream_t st) {
+    return with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        batch_init_kernel<T, NR><<<bd.tiles, kBlock, 0, st>>>(bd, res, v.d, v.r, v.p, v.part_init);
+        return hipGetLastError();
+    });
+}
+
+template <typename T>
+hipError_t batch_refine(const BatchDev &bd, double *x, const T *d, hipStream_t st) {
+    const long long count = (long long)bd.rows * bd.nr;
+    batch_refine_kernel<T><<<grid_for(count), kBlock, 0, st>>>(x, d, count);
+    return hipGetLastError();
+}
+
+hipError_t batch_residual(const BatchDev &bd, const double *val, const double *b, const double *x,
+                          double *res, double *part, double *sys_out, hipStream_t st) {
+    hipError_t e = with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        return with_lpr(bd.lpr, [&](auto lc) {
+            constexpr int LPR = decltype(lc)::value;
+            batch_residual_kernel<NR, LPR><<<bd.tiles, kBlock, 0, st>>>(bd, val, b, x, res, part);
+            return hipGetLastError();
+        });
+    });
+    if (e != hipSuccess) return e;
+    return with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        batch_system_sum_kernel<NR><<<bd.nsys, kBlock, 0, st>>>(bd, part, sys_out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t batch_narrow(const double *in, float *out, long long count, hipStream_t st) {
+    batch_narrow_kernel<<<grid_for(count), kBlock, 0, st>>>(in, out, count);
+    return hipGetLastError();
+}
+
+template hipError_t batch_cg_step<float>(const BatchDev &, const BatchVecs<float> &, int, int,
+                                         hipStream_t);
+template hipError_t batch_cg_step<double>(const BatchDev &, const BatchVecs<double> &, int, int,
+                                          hipStream_t);
+template hipError_t batch_spmm<float>(const BatchDev &, const BatchVecs<float> &, hipStream_t);
+template hipError_t batch_spmm<double>(const BatchDev &, const BatchVecs<double> &, hipStream_t);
+template hipError_t batch_init<float>(const BatchDev &, const BatchVecs<float> &, const double *,
+                                      hipStream_t);
+template hipError_t batch_init<double>(const BatchDev &, const BatchVecs<double> &,
+                                       const double *, hipStream_t);
+template hipError_t batch_refine<float>(const BatchDev &, double *, const float *, hipStream_t);
+template hipError_t batch_refine<double>(const BatchDev &, double *, const double *, hipStream_t);
+
+}  // namespace npbcg_dev

@@ -82,6 +82,10 @@ void set_generator_threads(int threads) { t_gen_threads = threads; }
 
 }  // namespace npbcg_detail
 
+extern "C" void npbcg_set_generator_threads(int threads) {
+    npbcg_detail::set_generator_threads(threads > 0 ? threads : 0);
+}
+
 extern "C" long npbcg_makea(int n, int nonzer, double rcond, double shift,
                             double seed, long nz_cap, int *rowstr, int *colidx,
                             double *a, double *ratio_out) {

@@ -57,6 +57,23 @@ MAX_PAIRS = 8
 NET_FEATURES = 8
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("nrhs", ctypes.c_int), ("mode", ctypes.c_int),
+                ("lanes_per_row", ctypes.c_int), ("rows_per_tile", ctypes.c_int),
+                ("inner_steps", ctypes.c_int), ("reserved", ctypes.c_int * 10)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("solves", ctypes.c_int), ("converged", ctypes.c_int), ("sweeps", ctypes.c_int),
+                ("max_steps", ctypes.c_int), ("lanes_per_row", ctypes.c_int),
+                ("rows_per_tile", ctypes.c_int), ("reserved_i", ctypes.c_int * 2),
+                ("time_ms", ctypes.c_double), ("max_relres", ctypes.c_double),
+                ("reserved_d", ctypes.c_double * 4)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
 class TrainOpts(ctypes.Structure):
     _fields_ = [("systems", ctypes.c_int), ("n_min", ctypes.c_int), ("n_max", ctypes.c_int),
                 ("nonzer_min", ctypes.c_int), ("nonzer_max", ctypes.c_int),
@@ -151,6 +168,17 @@ def _load() -> ctypes.CDLL:
         "npbcg_net_loss_grad": (ctypes.c_int, [vp, vp, _DP, vp, vp]),
         "npbcg_net_step": (ctypes.c_int, [vp, vp, ctypes.c_float, _DP]),
         "npbcg_net_apply": (ctypes.c_int, [vp, vp, _DP]),
+        "npbcg_set_generator_threads": (None, [ctypes.c_int]),
+        "npbcg_batch_default_opts": (None, [ctypes.POINTER(BatchOpts)]),
+        "npbcg_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, vp, vp, vp, vp,
+                                              ctypes.POINTER(BatchOpts)]),
+        "npbcg_batch_destroy": (None, [vp]),
+        "npbcg_batch_set_rhs": (ctypes.c_int, [vp, _F64P]),
+        "npbcg_batch_solve": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int,
+                                             ctypes.POINTER(BatchStats), vp, vp]),
+        "npbcg_batch_get_solution": (ctypes.c_int, [vp, _F64P]),
+        "npbcg_batch_tune": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int), _DP]),
         "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
         "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
                                           _F64P, _F64P, _DP]),
@@ -465,3 +493,99 @@ class Net:
         ms = ctypes.c_double(0.0)
         _check(LIB.npbcg_net_apply(self._h, ctx._h, ctypes.byref(ms)), "npbcg_net_apply")
         return ms.value
+
+
+def set_generator_threads(threads: int):
+    """Generator threads for npbcg_makea calls made from the calling thread."""
+    LIB.npbcg_set_generator_threads(threads)
+
+
+def make_many(params, workers: int = 0):
+    """makea for a list of (n, nonzer, shift, seed[, rcond]) tuples, one system
+    per host thread (each single-threaded)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(p):
+        set_generator_threads(1)
+        return makea(*p)
+
+    workers = workers or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, params))
+
+
+class Batch:
+    """Independent systems solved together (npbcg_batch_*)."""
+
+    def __init__(self, mats, nrhs: int = 1, mode: int = SOLVE_MIXED, device: int = 0,
+                 lanes_per_row: int = 0, rows_per_tile: int = 0, inner_steps: int = 0):
+        self.mats = list(mats)
+        self.nrhs = nrhs
+        self.n = np.array([m.n for m in self.mats], np.int32)
+        o = BatchOpts()
+        LIB.npbcg_batch_default_opts(ctypes.byref(o))
+        o.device, o.nrhs, o.mode = device, nrhs, mode
+        o.lanes_per_row, o.rows_per_tile, o.inner_steps = lanes_per_row, rows_per_tile, inner_steps
+        k = len(self.mats)
+        rs = (ctypes.c_void_p * k)(*[m.rowstr.ctypes.data for m in self.mats])
+        ci = (ctypes.c_void_p * k)(*[m.colidx.ctypes.data for m in self.mats])
+        av = (ctypes.c_void_p * k)(*[m.a.ctypes.data for m in self.mats])
+        self._h = ctypes.c_void_p()
+        _check(LIB.npbcg_batch_create(ctypes.byref(self._h), k, self.n.ctypes.data,
+                                      ctypes.cast(rs, ctypes.c_void_p),
+                                      ctypes.cast(ci, ctypes.c_void_p),
+                                      ctypes.cast(av, ctypes.c_void_p), ctypes.byref(o)),
+               "npbcg_batch_create")
+        self.rows = int(self.n.sum())
+
+    def close(self):
+        if self._h:
+            LIB.npbcg_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rhs(self, rhs):
+        """rhs: list (one per system) of [n_s, nrhs] arrays."""
+        b = np.concatenate([np.asarray(r, np.float64).reshape(-1, self.nrhs) for r in rhs])
+        _check(LIB.npbcg_batch_set_rhs(self._h, np.ascontiguousarray(b).reshape(-1)),
+               "npbcg_batch_set_rhs")
+
+    def tune(self, cache_path=None):
+        lpr, rt = ctypes.c_int(0), ctypes.c_int(0)
+        ms = ctypes.c_double(0.0)
+        _check(LIB.npbcg_batch_tune(self._h, None if cache_path is None else cache_path.encode(),
+                                    ctypes.byref(lpr), ctypes.byref(rt), ctypes.byref(ms)),
+               "npbcg_batch_tune")
+        return lpr.value, rt.value, ms.value
+
+    def solve(self, rtol: float = 1e-10, maxit: int = 2000, check: bool = True):
+        st = BatchStats()
+        ns = len(self.mats) * self.nrhs
+        steps = np.zeros(ns, np.int32)
+        rel = np.zeros(ns)
+        rc = LIB.npbcg_batch_solve(self._h, rtol, maxit, ctypes.byref(st), steps.ctypes.data,
+                                   rel.ctypes.data)
+        if check:
+            _check(rc, "npbcg_batch_solve")
+        return st, steps.reshape(-1, self.nrhs), rel.reshape(-1, self.nrhs)
+
+    def solution(self):
+        x = np.zeros(self.rows * self.nrhs)
+        _check(LIB.npbcg_batch_get_solution(self._h, x), "npbcg_batch_get_solution")
+        x = x.reshape(-1, self.nrhs)
+        out, off = [], 0
+        for n in self.n:
+            out.append(x[off:off + n])
+            off += n
+        return out

@@ -1,0 +1,92 @@
+"""Batched solves on the GPU: every (system, right-hand side) solution is
+checked against the oracle's FP64 CG solve of that system, and by its FP64
+true residual recomputed on the host."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def _ocsr(m):
+    return oracle.CSR(m.n, m.rowstr, m.colidx, m.a, m.ratio, m.nonzer, m.shift, m.rcond, m.seed)
+
+
+def _systems(gpu, count=5, seed=3):
+    rng = np.random.default_rng(seed)
+    params = [(int(rng.integers(1500, 5000)), int(rng.integers(7, 11)), 20.0, float(s + 1))
+              for s in range(count)]
+    return gpu.make_many(params, workers=4)
+
+
+def _rhs(mats, nrhs):
+    return [np.stack([oracle.make_rhs(m.n, 1000.0 * s + k) for k in range(nrhs)], axis=1)
+            for s, m in enumerate(mats)]
+
+
+@pytest.mark.parametrize("mode,nrhs", [(1, 4), (0, 4), (1, 1), (0, 2)])
+def test_batch_solves_match_oracle(gpu, mode, nrhs):
+    mats = _systems(gpu)
+    rhs = _rhs(mats, nrhs)
+    with gpu.Batch(mats, nrhs=nrhs, mode=mode) as bt:
+        bt.set_rhs(rhs)
+        st, steps, rel = bt.solve(RTOL, 3000)
+        xs = bt.solution()
+    print(f"mode {mode} nrhs {nrhs}: {st.as_dict()}; steps {steps.ravel().tolist()}")
+    assert st.converged == len(mats) * nrhs and st.max_relres <= RTOL
+    for s, m in enumerate(mats):
+        om = _ocsr(m)
+        for k in range(nrhs):
+            b = rhs[s][:, k]
+            x = xs[s][:, k]
+            host = np.linalg.norm(b - oracle.spmv(om, x)) / np.linalg.norm(b)
+            assert host <= RTOL and abs(host - rel[s, k]) <= 1e-3 * RTOL + 0.05 * host
+            ox, oit, _ = oracle.cg_solve(om, b, rtol=RTOL, maxit=5000)
+            assert np.max(np.abs(x - ox)) <= 1e-8 * np.max(np.abs(ox))
+            if mode == 0:   # FP64 batch steps follow the oracle's CG closely
+                assert abs(int(steps[s, k]) - oit) <= 3
+
+
+def test_batch_launch_shapes_agree(gpu):
+    mats = _systems(gpu, 3, seed=9)
+    rhs = _rhs(mats, 4)
+    sols = []
+    for lpr, rt in [(4, 64), (16, 256), (64, 1024)]:
+        with gpu.Batch(mats, nrhs=4, mode=0, lanes_per_row=lpr, rows_per_tile=rt) as bt:
+            bt.set_rhs(rhs)
+            bt.solve(RTOL, 3000)
+            sols.append(np.concatenate(bt.solution()))
+    for x in sols[1:]:
+        assert np.max(np.abs(x - sols[0])) <= 1e-9 * np.max(np.abs(sols[0]))
+
+
+def test_batch_autotuner_persists(gpu, tmp_path):
+    mats = _systems(gpu, 4, seed=5)
+    cache = str(tmp_path / "tune.txt")
+    with gpu.Batch(mats, nrhs=4) as bt:
+        lpr, rt, ms = bt.tune(cache)
+        bt.set_rhs(_rhs(mats, 4))
+        st, _, _ = bt.solve(RTOL, 3000)
+    assert lpr in (4, 8, 16, 32, 64) and 64 <= rt <= 4096 and ms > 0
+    assert st.lanes_per_row == lpr and st.rows_per_tile == rt and st.converged == st.solves
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and lines[0].split()[1:3] == [str(lpr), str(rt)]
+    with gpu.Batch(mats, nrhs=4) as bt:     # same shape: the record is reused, not re-timed
+        assert bt.tune(cache)[:2] == (lpr, rt)
+    assert len(open(cache).read().splitlines()) == 1
+
+
+def test_batch_reports_unconverged_and_errors(gpu):
+    mats = _systems(gpu, 2, seed=4)
+    with gpu.Batch(mats, nrhs=1, mode=0) as bt:
+        with pytest.raises(gpu.NpbcgError) as e:
+            bt.solve(RTOL, 10)
+        assert e.value.code == gpu.ERR_INVALID        # no right-hand side yet
+        bt.set_rhs(_rhs(mats, 1))
+        st, steps, rel = bt.solve(RTOL, 3, check=False)
+        assert st.converged < st.solves and np.all(steps <= 3)
+    with pytest.raises(gpu.NpbcgError):
+        gpu.Batch(mats, nrhs=3)
