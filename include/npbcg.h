@@ -190,7 +190,8 @@ enum { NPBCG_SOLVE_FP64 = 0, NPBCG_SOLVE_MIXED = 1 };
  * sparse form on the K = 4 largest off-diagonals |a_jk| of every row:
  *   M^-1 = S D^-1 S - Q (N + N^T) Q,  N_jk = a_jk / (a_jj a_kk),
  *   S = diag(exp g), Q = diag(exp(g + h)),
- * applied as one SELL-64 product per CG step. */
+ * applied as one CSR product (8 lanes per row) per CG step; the stop test
+ * of these two stays on the recursive r.r, as in plain CG. */
 enum {
     NPBCG_PRECOND_NONE = 0,     /* plain CG */
     NPBCG_PRECOND_JACOBI = 1,   /* M^-1 = 1 / diag(A) */
@@ -259,7 +260,9 @@ typedef struct npbcg_train_opts {
     int systems;             /* training systems */
     int n_min, n_max;        /* order n, uniform */
     int nonzer_min, nonzer_max;   /* nonzeros per generator vector, uniform */
-    double shift_min, shift_max;  /* diagonal shift, log-uniform */
+    double shift_min, shift_max;  /* diagonal shift = (nonzer+1)^2 * U[shift_min, shift_max]
+                                     (the NPB classes sit near 0.43 .. 0.2; the range keeps
+                                     every system definite) */
     double rcond;
     unsigned long long seed; /* seeds every system's (n, nonzer, shift, matrix seed, b) */
     int pairs;               /* residual/correction pairs per system (<= 8) */
@@ -271,15 +274,17 @@ typedef struct npbcg_train_opts {
 } npbcg_train_opts;
 
 typedef struct npbcg_train_stats {
-    int systems;
+    int systems;             /* systems trained on */
     int steps;
+    int skipped;             /* systems whose plain CG missed 1e-12 (not used) */
+    int reserved_i;
     double loss_first;       /* mean loss of the first 10% of systems, before their step */
     double loss_last;        /* same, last 10% */
     double base_first;       /* Neumann (g = h = 0) loss of the same systems */
     double base_last;
     double wait_ms;          /* time the device side waited for the generator */
     double total_ms;
-    double reserved[4];
+    double reserved[3];
 } npbcg_train_stats;
 
 int npbcg_net_param_count(void);

@@ -178,6 +178,8 @@ void refresh_problem(npbcg_ctx *c) {
     pb.part.nrm = pp + 3 * nv;           // 2 * nv
     pb.part.pq = pp + 5 * nv;            // ns
     pb.part.res = pp + 5 * nv + ns;      // 3 * ns
+    pb.part.rz[0] = pp + 5 * nv + 4 * ns;
+    pb.part.rz[1] = pp + 6 * nv + 4 * ns;
     pb.sc = c->d_sc;
     pb.zeta_hist = c->d_hist;
     pb.rnorm_hist = c->d_hist ? c->d_hist + c->hist_cap : nullptr;
@@ -472,7 +474,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     c->part_spmv = variant == npbcg_dev::kSpmvTiled ? std::max(sd.grid, vec_blocks)
                                                     : spmv_blocks;
     CREATE_HIP(hipMalloc(&c->d_part,
-                         sizeof(double) * (5 * (size_t)c->part_vec + 4 * (size_t)c->part_spmv)));
+                         sizeof(double) * (7 * (size_t)c->part_vec + 4 * (size_t)c->part_spmv)));
     if (variant == npbcg_dev::kSpmvTiled) {
         CREATE_HIP(npbcg_dev::prepare_sell_kernels(sd.lds_bytes));
         const size_t nwg1 = sm.wg_slice.size(), nsl1 = sm.slice_off.size();
@@ -843,7 +845,7 @@ int ensure_mixed(npbcg_ctx *c) {
     NPBCG_HIP(hipMalloc(&c->d_val32, sizeof(float) * stored));
     NPBCG_HIP(hipMalloc(&c->d_vec32, sizeof(float) * 4 * c->vstride));
     NPBCG_HIP(hipMalloc(&c->d_rows32, sizeof(float) * (size_t)sd.nblk * c->n));
-    NPBCG_HIP(hipMalloc(&c->d_part32, sizeof(double) * (3 * vb + (size_t)sd.grid)));
+    NPBCG_HIP(hipMalloc(&c->d_part32, sizeof(double) * (5 * vb + (size_t)sd.grid)));
     NPBCG_HIP(hipMalloc(&c->d_sc32, npbcg_dev::scalars_bytes()));
     NPBCG_HIP(hipMemsetAsync(c->d_vec32, 0, sizeof(float) * 4 * c->vstride, c->stream));
     NPBCG_HIP(hipMemsetAsync(c->d_sc32, 0, npbcg_dev::scalars_bytes(), c->stream));
@@ -863,6 +865,8 @@ int ensure_mixed(npbcg_ctx *c) {
     mp.part_rr[0] = c->d_part32 + vb;
     mp.part_rr[1] = c->d_part32 + 2 * vb;
     mp.part_pq = c->d_part32 + 3 * vb;
+    mp.part_rz[0] = c->d_part32 + 3 * vb + sd.grid;
+    mp.part_rz[1] = c->d_part32 + 4 * vb + sd.grid;
     mp.sc = c->d_sc32;
     mp.lds_bytes = lds32;
     NPBCG_HIP(hipStreamSynchronize(c->stream));
@@ -917,9 +921,9 @@ int ensure_precond_buffers(npbcg_ctx *c) {
     return NPBCG_OK;
 }
 
-// The top-K graph (device) and the SELL-64 layout of the sparse operator
-// M^-1: row j holds its diagonal, its K out-neighbours and its in-neighbours
-// (rows whose top-K contain j), padded to the longest row of its slice.
+// The top-K graph (device) and the CSR layout of the sparse operator M^-1:
+// row j holds its diagonal, its K out-neighbours and its in-neighbours (rows
+// whose top-K contain j).
 int ensure_row_graph(npbcg_ctx *c) {
     auto &g = c->g;
     if (g.built) return NPBCG_OK;
@@ -959,36 +963,25 @@ int ensure_row_graph(npbcg_ctx *c) {
                                  hipMemcpyDeviceToHost, c->stream));
         NPBCG_HIP(hipStreamSynchronize(c->stream));
     }
-    const int W = npbcg_dev::kWave;
-    const int slices = (n + W - 1) / W;
-    g.h_off.assign(slices + 1, 0);
-    for (int s = 0; s < slices; s++) {
-        int len = 1;
-        for (int j = s * W; j < std::min(n, (s + 1) * W); j++)
-            len = std::max(len, 1 + K + tptr[j + 1] - tptr[j]);
-        g.h_off[s + 1] = g.h_off[s] + len * W;
-    }
-    const size_t slots = (size_t)g.h_off[slices];
+    // row j: [diagonal | K out-neighbours | in-neighbours]
+    g.h_off.assign(n + 1, 0);
+    for (int j = 0; j < n; j++) g.h_off[j + 1] = g.h_off[j] + 1 + K + tptr[j + 1] - tptr[j];
+    const size_t slots = (size_t)g.h_off[n];
     std::vector<int> col(slots, 0);
     std::vector<float> src(slots, 0.0f);
-    for (int s = 0; s < slices; s++) {
-        const int len = (g.h_off[s + 1] - g.h_off[s]) / W;
-        for (int r = 0; r < W; r++) {
-            const int j = s * W + r;
-            if (j >= n) break;
-            auto slot = [&](int t) { return (size_t)g.h_off[s] + (size_t)t * W + r; };
-            for (int t = 0; t < len; t++) col[slot(t)] = j;  // padding reads r_j, value 0
-            for (int t = 0; t < K; t++) {
-                col[slot(1 + t)] = nbr[(size_t)j * K + t];
-                src[slot(1 + t)] = nval[(size_t)j * K + t];
-            }
-            for (int e = tptr[j]; e < tptr[j + 1]; e++) {
-                col[slot(1 + K + e - tptr[j])] = tsrc[e];
-                src[slot(1 + K + e - tptr[j])] = tval[e];
-            }
+    for (int j = 0; j < n; j++) {
+        size_t at = (size_t)g.h_off[j];
+        col[at++] = j;
+        for (int t = 0; t < K; t++, at++) {
+            col[at] = nbr[(size_t)j * K + t];
+            src[at] = nval[(size_t)j * K + t];
+        }
+        for (int e = tptr[j]; e < tptr[j + 1]; e++, at++) {
+            col[at] = tsrc[e];
+            src[at] = tval[e];
         }
     }
-    NPBCG_HIP(hipMalloc(&g.op_off, sizeof(int) * (slices + 1)));
+    NPBCG_HIP(hipMalloc(&g.op_off, sizeof(int) * (n + 1)));
     NPBCG_HIP(hipMalloc(&g.op_col, sizeof(int) * slots));
     NPBCG_HIP(hipMalloc(&g.op_src, sizeof(float) * slots));
     NPBCG_HIP(hipMalloc(&g.op_val, sizeof(double) * slots));
@@ -997,7 +990,7 @@ int ensure_row_graph(npbcg_ctx *c) {
     NPBCG_HIP(hipMalloc(&g.gh_learned, sizeof(float) * 2 * n));
     NPBCG_HIP(hipMalloc(&g.pz, sizeof(double) * c->vstride));
     NPBCG_HIP(hipMalloc(&g.pz32, sizeof(float) * c->vstride));
-    NPBCG_HIP(hipMemcpyAsync(g.op_off, g.h_off.data(), sizeof(int) * (slices + 1),
+    NPBCG_HIP(hipMemcpyAsync(g.op_off, g.h_off.data(), sizeof(int) * (n + 1),
                              hipMemcpyHostToDevice, c->stream));
     NPBCG_HIP(hipMemcpyAsync(g.op_col, col.data(), sizeof(int) * slots, hipMemcpyHostToDevice,
                              c->stream));
@@ -1009,7 +1002,6 @@ int ensure_row_graph(npbcg_ctx *c) {
     NPBCG_HIP(hipMemsetAsync(g.pz32, 0, sizeof(float) * c->vstride, c->stream));
     NPBCG_HIP(hipStreamSynchronize(c->stream));  // host vectors go out of scope
     g.op.n = n;
-    g.op.slices = slices;
     g.op.off = g.op_off;
     g.op.col = g.op_col;
     g.op.src_val = g.op_src;
@@ -1143,14 +1135,14 @@ int npbcg_get_preconditioner(npbcg_ctx *c, double *minv) {
                                  c->stream));
         NPBCG_HIP(hipStreamSynchronize(c->stream));
         std::memcpy(minv, c->h_stage, bytes);
-    } else if (c->pbs.pop) {  // slot 0 of every row of the SELL-64 operator
+    } else if (c->pbs.pop) {  // entry 0 of every row of the operator
         const auto &g = c->g;
         std::vector<double> val(g.h_off.back());
         NPBCG_HIP(hipMemcpyAsync(val.data(), g.op_val, sizeof(double) * val.size(),
                                  hipMemcpyDeviceToHost, c->stream));
         NPBCG_HIP(hipStreamSynchronize(c->stream));
         for (int j = 0; j < c->n; j++)
-            minv[j] = val[(size_t)g.h_off[j / npbcg_dev::kWave] + j % npbcg_dev::kWave];
+            minv[j] = val[(size_t)g.h_off[j]];
     } else {
         for (int j = 0; j < c->n; j++) minv[j] = 1.0;
     }

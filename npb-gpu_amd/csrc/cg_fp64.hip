@@ -266,19 +266,22 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     double *__restrict__ r, int n, const double *__restrict__ rho_part, int n_rho,
     const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
     const double *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
-    const double *__restrict__ minv) {
+    const double *__restrict__ minv, const double *__restrict__ tol_part) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rho_part, n_rho);
     const double d = reduce_partials<kBlock>(pq_part, n_pq);
     const double alpha = rho / d;
+    // the stop level: rtol^2 times this start's r0.r0 (tol_part), or times
+    // rho itself (plain CG, diagonal PCG)
+    const double rho_tol = (first_step && tol_part) ? reduce_partials<kBlock>(tol_part, n_rho) : rho;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->rho0 = rho;
         sc->d = d;
         sc->alpha = alpha;
         if (first_step) {
-            sc->tol2 = sc->rtol2 * fabs(rho);
-            sc->bb = rho;
-        }  // rho = r0.r0 of this start
+            sc->tol2 = sc->rtol2 * fabs(rho_tol);
+            sc->bb = rho_tol;
+        }
         sc->steps += 1;
     }
     double acc = 0.0;
@@ -300,24 +303,27 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
 }
 
 // p_out = r + beta p_in with beta = (new r.r) / (old r.r), both from
-// partials (p_in may equal p_out).
+// partials (p_in may equal p_out).  The stop test uses stop_part (the r.r of a
+// sparse-preconditioned solve) when given, else the new rho.
 __global__ __launch_bounds__(kBlock) void update_p_kernel(
     const double *__restrict__ r, const double *p_in, double *p_out, int n,
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
-    Scalars *sc, const double *__restrict__ minv) {
+    Scalars *sc, const double *__restrict__ minv, const double *__restrict__ stop_part) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rr_part, n_vec);
     const double rho0 = reduce_partials<kBlock>(rho_part, n_vec);
     const double beta = rho / rho0;
+    const double level = stop_part ? reduce_partials<kBlock>(stop_part, n_vec) : rho;
+    const bool stop = sc->stop_enabled && fabs(level) <= sc->tol2;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->rho = rho;
         sc->beta = beta;
         // Solve mode: the recursive residual met the tolerance; the remaining
         // steps of the launched sequence return at entry.  p is left as is
         // (the solution z is final).
-        if (sc->stop_enabled && fabs(rho) <= sc->tol2) sc->done = 1;
+        if (stop) sc->done = 1;
     }
-    if (sc->stop_enabled && fabs(rho) <= sc->tol2) return;
+    if (stop) return;
     if (minv) {
         for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
             p_out[j] = minv[j] * r[j] + beta * p_in[j];
@@ -600,27 +606,34 @@ hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st) {
                                                      scale_x, pb.sc, pb.part.xx, pb.minv);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !pb.pop) return e;
-    // sparse M^-1: p = z = M^-1 r, rho = r.z
-    return launch_precond_apply(*pb.pop, pb.r, pb.pz, pb.p, pb.part.xx, pb.vec_blocks, nullptr,
-                                st);
+    // sparse M^-1: p = z = M^-1 r; r.z into rz[1] (read by step 0), r.r stays in xx
+    return launch_precond_apply(*pb.pop, pb.r, pb.pz, pb.p, pb.part.rz[1], pb.vec_blocks,
+                                nullptr, st);
 }
 
 hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
                             int sets_tol) {
+    // sparse preconditioner: alpha = (r.z of the previous step) / p.q, where
+    // the start's r0.z0 sits in rz[1]; the stop level comes from r0.r0 (xx)
     update_zr_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
-        p_of_step(pb, step), pb.q, pb.z, pb.r, pb.n, rho_source(pb, step), pb.vec_blocks,
-        pb.part.pq,
+        p_of_step(pb, step), pb.q, pb.z, pb.r, pb.n,
+        pb.pop ? pb.part.rz[(step + 1) & 1] : rho_source(pb, step), pb.vec_blocks, pb.part.pq,
         spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part,
         pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc,
-        (step == 0 && sets_tol) ? 1 : 0, pb.minv);
+        (step == 0 && sets_tol) ? 1 : 0, pb.minv, pb.pop ? pb.part.xx : nullptr);
     return hipGetLastError();
 }
 
 // p = r + beta p closing CG step `step`; the result always lands in pb.p.
 hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st) {
-    update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
-        pb.pop ? pb.pz : pb.r, p_of_step(pb, step), pb.p, pb.n, pb.part.rr[step & 1],
-        rho_source(pb, step), pb.vec_blocks, pb.sc, pb.minv);
+    if (pb.pop)  // beta from r.z; stop test on r.r
+        update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+            pb.pz, p_of_step(pb, step), pb.p, pb.n, pb.part.rz[step & 1],
+            pb.part.rz[(step + 1) & 1], pb.vec_blocks, pb.sc, nullptr, pb.part.rr[step & 1]);
+    else
+        update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+            pb.r, p_of_step(pb, step), pb.p, pb.n, pb.part.rr[step & 1], rho_source(pb, step),
+            pb.vec_blocks, pb.sc, pb.minv, nullptr);
     return hipGetLastError();
 }
 
@@ -682,8 +695,8 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
 hipError_t enqueue_solve_step(const Fp64Problem &pb, int it, int sets_tol, hipStream_t st) {
     hipError_t e = launch_spmv<0>(pb, st, nullptr, nullptr, it);
     if (e == hipSuccess) e = launch_update_zr(pb, it, st, sets_tol);
-    if (e == hipSuccess && pb.pop)  // z = M^-1 r; its r.z replaces the r.r partials
-        e = launch_precond_apply(*pb.pop, pb.r, pb.pz, nullptr, pb.part.rr[it & 1],
+    if (e == hipSuccess && pb.pop)  // z = M^-1 r and its r.z partials
+        e = launch_precond_apply(*pb.pop, pb.r, pb.pz, nullptr, pb.part.rz[it & 1],
                                  pb.vec_blocks, pb.sc, st);
     if (e == hipSuccess) e = launch_update_p(pb, it, st);
     return e;

@@ -42,6 +42,8 @@ struct Partials {
     double *pq;      // p.q of the SpMV           [spmv_grid]
     double *res;     // ||x-Az||^2, x.z, z.z of the residual [3 * max(spmv_grid, vec_blocks)]
     double *nrm;     // x.z, z.z of norms_kernel  [2 * vec_blocks]
+    double *rz[2];   // r.M^-1 r of the sparse preconditioner, by CG step; [1] also
+                     // holds the start's r0.z0 [vec_blocks]
 };
 
 struct PrecondOp;  // sparse preconditioner operator (learned_kernels.h)
@@ -135,6 +137,7 @@ struct MixedProblem {
     float *rows_part = nullptr;   // nblk * n per-block row sums
     double *part_xx = nullptr;    // r0.r0 partials of an inner start [vec_blocks]
     double *part_rr[2] = {nullptr, nullptr};  // r.r partials [vec_blocks]
+    double *part_rz[2] = {nullptr, nullptr};  // r.M^-1 r partials (sparse preconditioner)
     double *part_pq = nullptr;    // p.q partials [sell grid]
     Scalars *sc = nullptr;        // inner-solve scalars
     int lds_bytes = 0;

@@ -117,18 +117,20 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
     float *__restrict__ r, int n, const double *__restrict__ rho_part,
     const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
     const float *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
-    const float *__restrict__ minv) {
+    const float *__restrict__ minv, const double *__restrict__ tol_part) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rho_part, gridDim.x);
     const double dd = reduce_partials<kBlock>(pq_part, n_pq);
     const double alpha = rho / dd;
+    // stop level from r0.r0 (tol_part, sparse preconditioner) or rho
+    const double rho_tol = (first_step && tol_part) ? reduce_partials<kBlock>(tol_part, gridDim.x) : rho;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->rho0 = rho;
         sc->d = dd;
         sc->alpha = alpha;
         if (first_step) {
-            sc->tol2 = sc->rtol2 * fabs(rho);
-            sc->bb = rho;
+            sc->tol2 = sc->rtol2 * fabs(rho_tol);
+            sc->bb = rho_tol;
         }
         sc->steps += 1;
     }
@@ -146,16 +148,18 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
     store_block_partial<kBlock>(acc, rr_out);
 }
 
-// beta = (new r.r) / (old r.r) in FP64; p = r + beta p; stop test.
+// beta = (new r.r) / (old r.r) in FP64; p = r + beta p; stop test (on
+// stop_part, the r.r of a sparse-preconditioned solve, when given).
 __global__ __launch_bounds__(kBlock) void update_p32_kernel(
     const float *__restrict__ r, float *__restrict__ p, int n,
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, Scalars *sc,
-    const float *__restrict__ minv) {
+    const float *__restrict__ minv, const double *__restrict__ stop_part) {
     if (sc->done) return;
     const double rho = reduce_partials<kBlock>(rr_part, gridDim.x);
     const double rho0 = reduce_partials<kBlock>(rho_part, gridDim.x);
     const double beta = rho / rho0;
-    const bool stop = fabs(rho) <= sc->tol2;
+    const double level = stop_part ? reduce_partials<kBlock>(stop_part, gridDim.x) : rho;
+    const bool stop = fabs(level) <= sc->tol2;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->rho = rho;
         sc->beta = beta;
@@ -242,8 +246,8 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
     init32_kernel<<<vb, kBlock, 0, st>>>(pb.x, pb.r, first_sweep, pb.n, mp.d, mp.r, mp.p,
                                          mp.part_xx, mp.minv);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess && mp.pop)  // p = z = M^-1 r, rho = r.z
-        e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, mp.p, mp.part_xx, vb, nullptr, st);
+    if (e == hipSuccess && mp.pop)  // p = z = M^-1 r; r0.z0 into rz[1], r0.r0 stays in xx
+        e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, mp.p, mp.part_rz[1], vb, nullptr, st);
     for (int it = 0; it < steps && e == hipSuccess; it++) {
         const SellDev &sd = pb.sell;
         if (sd.block == 512)
@@ -256,20 +260,26 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
                 mp.part_pq, pb.n, sd.W, sd.nrange, mp.sc);
         e = hipGetLastError();
         if (e != hipSuccess) break;
-        update_zr32_kernel<<<vb, kBlock, 0, st>>>(mp.p, mp.q, mp.d, mp.r, pb.n,
-                                                  rho32_source(mp, it), mp.part_pq, sd.grid,
-                                                  mp.part_rr[it & 1], mp.rows_part, sd.nblk,
-                                                  mp.sc, it == 0 ? 1 : 0, mp.minv);
+        // sparse preconditioner: alpha from the previous step's r.z (the
+        // start's in rz[1]); beta from r.z; the stop test on r.r
+        const double *rho_prev = mp.pop ? mp.part_rz[(it + 1) & 1] : rho32_source(mp, it);
+        update_zr32_kernel<<<vb, kBlock, 0, st>>>(mp.p, mp.q, mp.d, mp.r, pb.n, rho_prev,
+                                                  mp.part_pq, sd.grid, mp.part_rr[it & 1],
+                                                  mp.rows_part, sd.nblk, mp.sc, it == 0 ? 1 : 0,
+                                                  mp.minv, mp.pop ? mp.part_xx : nullptr);
         e = hipGetLastError();
         if (e != hipSuccess) break;
         if (mp.pop) {
-            e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, nullptr, mp.part_rr[it & 1], vb,
+            e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, nullptr, mp.part_rz[it & 1], vb,
                                        mp.sc, st);
             if (e != hipSuccess) break;
+            update_p32_kernel<<<vb, kBlock, 0, st>>>(mp.pz, mp.p, pb.n, mp.part_rz[it & 1],
+                                                     rho_prev, mp.sc, nullptr,
+                                                     mp.part_rr[it & 1]);
+        } else {
+            update_p32_kernel<<<vb, kBlock, 0, st>>>(mp.r, mp.p, pb.n, mp.part_rr[it & 1],
+                                                     rho_prev, mp.sc, mp.minv, nullptr);
         }
-        update_p32_kernel<<<vb, kBlock, 0, st>>>(mp.pop ? mp.pz : mp.r, mp.p, pb.n,
-                                                 mp.part_rr[it & 1], rho32_source(mp, it), mp.sc,
-                                                 mp.minv);
         e = hipGetLastError();
     }
     return e;
@@ -285,8 +295,8 @@ hipError_t launch_residual_restart(const Fp64Problem &pb, hipStream_t st) {
                                                               pb.part.xx, pb.minv);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !pb.pop) return e;
-    return launch_precond_apply(*pb.pop, pb.r, pb.pz, pb.p, pb.part.xx, pb.vec_blocks, nullptr,
-                                st);
+    return launch_precond_apply(*pb.pop, pb.r, pb.pz, pb.p, pb.part.rz[1], pb.vec_blocks,
+                                nullptr, st);
 }
 
 hipError_t launch_narrow(const double *in, float *out, int n, int blocks, hipStream_t st) {

@@ -119,14 +119,13 @@ hipError_t launch_scale_inverse(float *g, int count, const double *den, hipStrea
 // sum of count doubles in fixed order (one workgroup) -> out[0]
 hipError_t launch_sum(const double *part, int count, double *out, hipStream_t st);
 
-// The preconditioner as a SELL-64 operator (slices of 64 rows, slot t of row
-// r of slice s at off[s] + t*64 + r).  Slot 0 of every row is its diagonal.
-// src_val holds N for off-diagonal slots (0 on padding), src_col the column.
+// The preconditioner as a CSR operator: row j holds its diagonal first, then
+// its K out-neighbours, then its in-neighbours (rows whose top-K contain j).
+// src_val holds N for off-diagonal entries (0 for padding self edges).
 struct PrecondOp {
     int n = 0;
-    int slices = 0;
-    const int *off = nullptr;      // [slices+1]; slots per row = (off[s+1]-off[s]) / 64
-    const int *col = nullptr;      // [off[slices]]
+    const int *off = nullptr;      // [n+1]
+    const int *col = nullptr;      // [off[n]]
     const float *src_val = nullptr;
     double *val = nullptr;         // FP64 values
     float *val32 = nullptr;        // FP32 values

@@ -766,59 +766,57 @@ __global__ __launch_bounds__(kBlock) void pack_weights_kernel(const float *__res
 // Preconditioner operator
 // ---------------------------------------------------------------------------
 
-// One thread per (slice, row): slot 0 is the diagonal exp(2 g_j) / a_jj,
-// off-diagonal slots -exp(g_j + h_j + g_k + h_k) N, padding 0.
+// One thread per row: entry 0 is the diagonal exp(2 g_j) / a_jj, the others
+// -exp(g_j + h_j + g_k + h_k) N (0 for padding).
 __global__ __launch_bounds__(kBlock) void precond_values_kernel(
-    int n, int slices, const int *__restrict__ off, const int *__restrict__ col,
+    int n, const int *__restrict__ off, const int *__restrict__ col,
     const float *__restrict__ src_val, const double *__restrict__ adiag,
     const float *__restrict__ gh, double *__restrict__ val, float *__restrict__ val32) {
-    const long long total = (long long)slices * kWave;
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < total;
-         i += (long long)gridDim.x * kBlock) {
-        const int s = (int)(i / kWave), r = (int)(i % kWave);
-        const int j = s * kWave + r;
-        const int base = off[s];
-        const int len = (off[s + 1] - base) / kWave;
-        for (int t = 0; t < len; t++) {
-            const int at = base + t * kWave + r;
-            double v = 0.0;
-            if (j < n) {
-                if (t == 0) {
-                    const double d = adiag[j];
-                    v = d != 0.0 ? exp(2.0 * (double)gh[j]) / d : 0.0;
-                } else if (src_val[at] != 0.0f) {
-                    const int k = col[at];
-                    v = -exp((double)gh[j] + gh[n + j] + gh[k] + gh[n + k]) * src_val[at];
-                }
-            }
-            val[at] = v;
-            val32[at] = (float)v;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const double d = adiag[j];
+        const double gj = (double)gh[j] + gh[n + j];
+        double v = d != 0.0 ? exp(2.0 * (double)gh[j]) / d : 0.0;
+        val[off[j]] = v;
+        val32[off[j]] = (float)v;
+        for (int e = off[j] + 1; e < off[j + 1]; e++) {
+            const float sv = src_val[e];
+            const int k = col[e];
+            v = sv != 0.0f ? -exp(gj + gh[k] + gh[n + k]) * sv : 0.0;
+            val[e] = v;
+            val32[e] = (float)v;
         }
     }
 }
 
-// z = M^-1 r over the SELL-64 operator; p = z when p is given; r.z partials.
+// z = M^-1 r, 8 lanes per row; p = z when p is given; r.z partials.  The
+// gathers are latency-bound, so the workgroups are 1024 wide: the partial
+// count stays that of the vector kernels while 4x more rows are in flight.
+constexpr int kApplyBlock = 1024;
 template <typename T>
-__global__ __launch_bounds__(kBlock) void precond_apply_kernel(
+__global__ __launch_bounds__(kApplyBlock) void precond_apply_kernel(
     int n, const int *__restrict__ off, const int *__restrict__ col, const T *__restrict__ val,
     const T *__restrict__ r, T *__restrict__ z, T *__restrict__ p, double *__restrict__ rz_part,
     const Scalars *__restrict__ sc) {
+    constexpr int kLanes = 8;
+    constexpr int kRows = kApplyBlock / kLanes;
     if (sc && sc->done) return;
+    const int sub = threadIdx.x % kLanes;
     double acc = 0.0;
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        const int s = j / kWave, rr = j % kWave;
-        const int base = off[s];
-        const int len = (off[s + 1] - base) / kWave;
+    const long long stride = (long long)gridDim.x * kRows;
+    for (long long j = blockIdx.x * (long long)kRows + threadIdx.x / kLanes; j < n;
+         j += stride) {
         T zj = 0;
-        for (int t = 0; t < len; t++) {
-            const int at = base + t * kWave + rr;
-            zj += val[at] * r[col[at]];
+        const int e1 = off[j + 1];
+        for (int e = off[j] + sub; e < e1; e += kLanes) zj += val[e] * r[col[e]];
+#pragma unroll
+        for (int m = kLanes / 2; m >= 1; m >>= 1) zj += __shfl_xor(zj, m, kWave);
+        if (sub == 0) {
+            z[j] = zj;
+            if (p) p[j] = zj;
+            acc += (double)r[j] * (double)zj;
         }
-        z[j] = zj;
-        if (p) p[j] = zj;
-        acc += (double)r[j] * (double)zj;
     }
-    store_block_partial<kBlock>(acc, rz_part);
+    store_block_partial<kApplyBlock>(acc, rz_part);
 }
 
 // ---------------------------------------------------------------------------
@@ -967,24 +965,24 @@ hipError_t launch_pack_weights(const float *params, __bf16 *wb, hipStream_t st) 
 
 hipError_t launch_precond_values(const PrecondOp &op, const double *adiag, const float *gh,
                                  hipStream_t st) {
-    precond_values_kernel<<<blocks_for((long long)op.slices * kWave), kBlock, 0, st>>>(
-        op.n, op.slices, op.off, op.col, op.src_val, adiag, gh, op.val, op.val32);
+    precond_values_kernel<<<blocks_for(op.n), kBlock, 0, st>>>(op.n, op.off, op.col, op.src_val,
+                                                               adiag, gh, op.val, op.val32);
     return hipGetLastError();
 }
 
 hipError_t launch_precond_apply(const PrecondOp &op, const double *r, double *z, double *p,
                                 double *rz_part, int blocks, const Scalars *sc,
                                 hipStream_t st) {
-    precond_apply_kernel<double><<<blocks, kBlock, 0, st>>>(op.n, op.off, op.col, op.val, r, z, p,
-                                                            rz_part, sc);
+    precond_apply_kernel<double><<<blocks, kApplyBlock, 0, st>>>(op.n, op.off, op.col, op.val, r,
+                                                                 z, p, rz_part, sc);
     return hipGetLastError();
 }
 
 hipError_t launch_precond_apply32(const PrecondOp &op, const float *r, float *z, float *p,
                                   double *rz_part, int blocks, const Scalars *sc,
                                   hipStream_t st) {
-    precond_apply_kernel<float><<<blocks, kBlock, 0, st>>>(op.n, op.off, op.col, op.val32, r, z,
-                                                           p, rz_part, sc);
+    precond_apply_kernel<float><<<blocks, kApplyBlock, 0, st>>>(op.n, op.off, op.col, op.val32,
+                                                                r, z, p, rz_part, sc);
     return hipGetLastError();
 }
 

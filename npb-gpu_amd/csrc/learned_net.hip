@@ -199,8 +199,8 @@ void system_params(const npbcg_train_opts &o, int i, GenSystem &g) {
     g.n = std::min(g.n, o.n_max);
     g.nonzer = o.nonzer_min + (int)(uniform01(s) * (o.nonzer_max - o.nonzer_min + 1));
     g.nonzer = std::min(g.nonzer, o.nonzer_max);
-    const double l0 = std::log(o.shift_min), l1 = std::log(o.shift_max);
-    g.shift = std::exp(l0 + uniform01(s) * (l1 - l0));
+    const double w = (double)(g.nonzer + 1) * (g.nonzer + 1);
+    g.shift = w * (o.shift_min + uniform01(s) * (o.shift_max - o.shift_min));
     g.seed = (double)(1 + splitmix64(s) % ((1ull << 46) - 2));  // integer in (0, 2^46)
     g.rhs_seed = splitmix64(s);
 }
@@ -237,8 +237,8 @@ void npbcg_net_default_train_opts(npbcg_train_opts *o) {
     o->n_max = 75000;
     o->nonzer_min = 7;
     o->nonzer_max = 15;
-    o->shift_min = 10.0;
-    o->shift_max = 120.0;
+    o->shift_min = 0.3;
+    o->shift_max = 0.5;
     o->rcond = 0.1;
     o->seed = 20260101ull;
     o->pairs = NPBCG_MAX_PAIRS;
@@ -472,6 +472,7 @@ int npbcg_net_train(npbcg_net *net, const npbcg_train_opts *opts, npbcg_train_st
     double wait_ms = 0.0;
     const int win = std::max(1, o.systems / 10);
     double lf = 0, ll = 0, bf = 0, bl = 0;
+    int nf = 0, nl = 0;
     int rc = NPBCG_OK;
     std::vector<double> b;
     for (int i = 0; i < o.systems && rc == NPBCG_OK; i++) {
@@ -501,6 +502,14 @@ int npbcg_net_train(npbcg_net *net, const npbcg_train_opts *opts, npbcg_train_st
         uniform_rhs(g.rhs_seed, g.n, b);
         rc = npbcg_set_rhs(ctx, b.data());
         if (rc == NPBCG_OK) rc = npbcg_log_pairs(ctx, o.pairs, nullptr, nullptr);
+        if (rc == NPBCG_ERR_NOCONV) {  // no x* for this system: leave it out
+            npbcg_destroy(ctx);
+            rc = NPBCG_OK;
+            s.skipped++;
+            if (loss_hist) loss_hist[i] = std::nan("");
+            if (base_hist) base_hist[i] = std::nan("");
+            continue;
+        }
         double loss = 0, base = 0;
         for (int k = 0; k < o.steps_per_system && rc == NPBCG_OK; k++) {
             rc = evaluate(net, ctx, true, k == 0, true);
@@ -523,19 +532,21 @@ int npbcg_net_train(npbcg_net *net, const npbcg_train_opts *opts, npbcg_train_st
         if (i < win) {
             lf += loss;
             bf += base;
+            nf++;
         }
         if (i >= o.systems - win) {
             ll += loss;
             bl += base;
+            nl++;
         }
         s.systems++;
     }
     shutdown();
     if (rc != NPBCG_OK) return rc;
-    s.loss_first = lf / win;
-    s.loss_last = ll / win;
-    s.base_first = bf / win;
-    s.base_last = bl / win;
+    s.loss_first = nf ? lf / nf : 0.0;
+    s.loss_last = nl ? ll / nl : 0.0;
+    s.base_first = nf ? bf / nf : 0.0;
+    s.base_last = nl ? bl / nl : 0.0;
     s.wait_ms = wait_ms;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = s;

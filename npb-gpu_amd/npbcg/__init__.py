@@ -86,13 +86,14 @@ class TrainOpts(ctypes.Structure):
 
 class TrainStats(ctypes.Structure):
     _fields_ = [("systems", ctypes.c_int), ("steps", ctypes.c_int),
+                ("skipped", ctypes.c_int), ("reserved_i", ctypes.c_int),
                 ("loss_first", ctypes.c_double), ("loss_last", ctypes.c_double),
                 ("base_first", ctypes.c_double), ("base_last", ctypes.c_double),
                 ("wait_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("reserved", ctypes.c_double * 4)]
+                ("reserved", ctypes.c_double * 3)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class SolveStats(ctypes.Structure):

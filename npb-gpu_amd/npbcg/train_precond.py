@@ -3,8 +3,9 @@
     python npb-gpu_amd/npbcg/train_precond.py [--systems 2000] [--out PATH]
 
 Defaults: 2000 systems, n uniform in [10000, 75000], 7 to 15 nonzeros per
-generator vector, shift log-uniform in [10, 120], rcond 0.1, 8 CG
-residual/correction pairs per system.  Training runs on the GPU through
+generator vector, shift (nonzer+1)^2 * U[0.3, 0.5] (every system definite;
+nonzer 15 gives shifts in [77, 128]), rcond 0.1, 8 CG residual/correction
+pairs per system.  Training runs on the GPU through
 npbcg_net_train (the library's own forward/backward kernels and Adam); the
 host only runs the matrix generator threads.  The parameters and feature
 normalisation go to an .npz file (default: learned_precond.npz next to
@@ -54,6 +55,7 @@ def main(argv=None) -> int:
     lh, bh = [], []
     t0 = time.perf_counter()
     wait_ms = 0.0
+    skipped = 0
     with npbcg.Net(seed=a.net_seed) as net:
         for first in range(0, a.systems, a.chunk):
             o.first_system = first
@@ -62,16 +64,18 @@ def main(argv=None) -> int:
             lh.append(l)
             bh.append(b)
             wait_ms += st.wait_ms
+            skipped += st.skipped
             print(f"systems {first + o.systems}/{a.systems}: mean loss/base "
-                  f"{np.mean(l / b):.4f} ({time.perf_counter() - t0:.1f} s)", flush=True)
+                  f"{np.nanmean(l / b):.4f} skipped {st.skipped} "
+                  f"({time.perf_counter() - t0:.1f} s)", flush=True)
         net.save(a.out)
     lh, bh = np.concatenate(lh), np.concatenate(bh)
     if a.history:
         np.savez(a.history, loss=lh, base=bh)
     w = max(1, a.systems // 10)
-    summary = dict(systems=int(lh.size), out=a.out,
-                   ratio_first=float(np.mean(lh[:w] / bh[:w])),
-                   ratio_last=float(np.mean(lh[-w:] / bh[-w:])),
+    summary = dict(systems=int(lh.size), skipped=skipped, out=a.out,
+                   ratio_first=float(np.nanmean(lh[:w] / bh[:w])),
+                   ratio_last=float(np.nanmean(lh[-w:] / bh[-w:])),
                    generator_wait_s=round(wait_ms / 1e3, 2),
                    total_s=round(time.perf_counter() - t0, 2))
     print(json.dumps(summary))
