@@ -10,7 +10,7 @@ cg.cu:491-522).  The matrix and vectors are resident in HBM before timing.
 value: whole-step algorithmic bytes / wall time (GB/s), with the algorithmic
 bytes of one outer iteration = 26 * B_spmv + 25 * 72 n + 56 n, where
 B_spmv = 12 nnz + 4 (n+1) + 16 n (matrix streamed once, vector read once,
-result written once) — see DESIGN.md.  The SpMV kernel's own duration is
+result written once).  The SpMV kernel's own duration is
 sampled with HIP events on the library's stream during the timed steps and
 gives the roofline entry.  cpu_baseline: the oracle's serial power loop
 (single thread) on a bounded sample of the same workload.
@@ -247,6 +247,69 @@ def solve_comparison(reps: int):
     return out
 
 
+BATCH_WORKLOAD = dict(workload="npb_cg_batch_64x4", systems=64, n_min=2000, n_max=40000,
+                      nonzer_min=7, nonzer_max=13, shift=20.0, rcond=0.1, nrhs=4, rtol=1e-10,
+                      maxit=3000)
+
+
+def batch_systems(bw=BATCH_WORKLOAD):
+    """(n, nonzer, shift, seed, rcond) of the batch: system s = 1..64 uses
+    matrix seed s, and n ~ U{n_min..n_max}, nonzer ~ U{nonzer_min..nonzer_max}
+    drawn (in that order) from numpy's PCG64 seeded s."""
+    out = []
+    for s in range(1, bw["systems"] + 1):
+        rng = np.random.default_rng(s)
+        n = int(rng.integers(bw["n_min"], bw["n_max"] + 1))
+        nz = int(rng.integers(bw["nonzer_min"], bw["nonzer_max"] + 1))
+        out.append((n, nz, bw["shift"], float(s), bw["rcond"]))
+    return out
+
+
+def batch_rhs(mats, nrhs):
+    """Right-hand sides uniform(-1, 1): numpy PCG64 seeded 1000 + s per system."""
+    return [np.random.default_rng(1000 + s).uniform(-1.0, 1.0, (m.n, nrhs))
+            for s, m in enumerate(mats, start=1)]
+
+
+def batch_comparison(reps: int):
+    """The 64-system batch with 4 right-hand sides each: every solve to a
+    1e-10 FP64-checked relative residual (or maxit), mixed precision and FP64,
+    with the SpMM launch shape chosen by the autotuner."""
+    import npbcg
+
+    bw = BATCH_WORKLOAD
+    t0 = time.perf_counter()
+    mats = npbcg.make_many(batch_systems(bw))
+    t_gen = time.perf_counter() - t0
+    rhs = batch_rhs(mats, bw["nrhs"])
+    nnz = int(sum(m.nnz for m in mats))
+    rows = int(sum(m.n for m in mats))
+    out = {"workload": bw["workload"], "systems": bw["systems"], "nrhs": bw["nrhs"],
+           "rows": rows, "nnz": nnz, "generate_seconds": round(t_gen, 2),
+           "nonzer_counts": {str(k): sum(1 for m in mats if m.nonzer == k)
+                             for k in range(bw["nonzer_min"], bw["nonzer_max"] + 1)}}
+    cache = os.path.join(ROOT, "npb-gpu_amd", "npbcg", "tune_records.txt")
+    for name, mode in (("mixed", npbcg.SOLVE_MIXED), ("fp64", npbcg.SOLVE_FP64)):
+        with npbcg.Batch(mats, nrhs=bw["nrhs"], mode=mode) as bt:
+            lpr, rt, ms = bt.tune(cache)
+            bt.set_rhs(rhs)
+            bt.solve(bw["rtol"], bw["maxit"], check=False)   # warm
+            times, st, steps = [], None, None
+            for _ in range(reps):
+                st, steps, rel = bt.solve(bw["rtol"], bw["maxit"], check=False)
+                times.append(st.time_ms)
+            val_b = 4 if mode == npbcg.SOLVE_MIXED else 8
+            spmm_bytes = nnz * (val_b + 4) + 4 * (rows + 1) + 2 * rows * bw["nrhs"] * val_b
+            d = st.as_dict()
+            d.update(time_ms_median=round(float(np.median(times)), 3),
+                     steps_median=float(np.median(steps)),
+                     unconverged_systems=sorted({i + 1 for i in np.nonzero(rel > bw["rtol"])[0]}),
+                     tuned_spmm_ms=round(ms, 4),
+                     tuned_spmm_gbs=round(spmm_bytes / (ms * 1e-3) / 1e9, 1))
+            out[name] = d
+    return out
+
+
 def rhs_for(n: int, seed: float) -> np.ndarray:
     """b_i = 2 u_i - 1, u_i successive draws of the 46-bit LCG (a = 5^13)
     from seed + 1 (the definition the oracle's orc_make_rhs restates)."""
@@ -275,7 +338,19 @@ def main():
                     help="short run for rocprofv3 counter passes")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="summarise two counter passes into the traffic file")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only the n=150000 time-to-1e-10 comparison (JSON on stdout)")
+    ap.add_argument("--batch-reps", type=int, default=3,
+                    help="repeats of the 64-system batched solve (0 = skip)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 64-system batched solve (JSON on stdout)")
     args = ap.parse_args()
+    if args.solve_only:
+        print(json.dumps(solve_comparison(max(1, args.solve_reps))))
+        return
+    if args.batch_only:
+        print(json.dumps(batch_comparison(max(1, args.batch_reps))))
+        return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
         return
@@ -284,8 +359,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the path is not sharded "
-                          "across GPUs (see DESIGN.md for how it would shard)",
+        print(json.dumps({"error": "one GPU only: the solver is not sharded across GPUs",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
@@ -356,6 +430,7 @@ def main():
         achieved / stream["copy_gbs_at_matrix_size"], 4)
 
     solve = solve_comparison(args.solve_reps) if args.solve_reps > 0 else None
+    batch = batch_comparison(args.batch_reps) if args.batch_reps > 0 else None
 
     cpu = None
     if rank == 0 and args.cpu_outer_iters > 0:
@@ -386,6 +461,7 @@ def main():
         "roofline": roofline,
         "stream_copy": stream,
         "solve_to_1e-10": solve,
+        "batched_solves": batch,
         "cpu_baseline": cpu,
     }
     ctx.close()

@@ -771,17 +771,23 @@ int npbcg_describe(const npbcg_ctx *c, char *buf, int len) {
     if (!c || (!buf && len > 0) || len < 0)
         return set_error(NPBCG_ERR_INVALID, "npbcg_describe: invalid argument");
     const npbcg_dev::SellDev &sd = c->pb.sell;
-    char tmp[512];
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) std::strcpy(prop.name, "unknown");
+    prop.name[sizeof(prop.name) - 1] = '\0';
+    for (char *q = prop.name; *q; q++)
+        if (*q == '"' || *q == '\\') *q = ' ';
+    char tmp[768];
     int k = std::snprintf(
         tmp, sizeof tmp,
         "{\"n\": %d, \"nnz\": %lld, \"spmv_variant\": %d, \"vec_blocks\": %d, "
         "\"csr_blocks\": %d, \"block_width\": %d, \"col_blocks\": %d, "
         "\"range_rows\": %d, \"row_ranges\": %d, \"sell_workgroups\": %d, "
         "\"sell_threads\": %d, \"sell_stored\": %lld, \"cgitmax\": %d, "
-        "\"use_graph\": %d, \"fuse_p_update\": %d}",
+        "\"use_graph\": %d, \"fuse_p_update\": %d, \"device_name\": \"%s\", "
+        "\"compute_units\": %d}",
         c->n, c->nnz, c->pb.spmv_variant, c->pb.vec_blocks, c->pb.spmv_blocks, sd.W,
         sd.nblk, sd.R, sd.nrange, sd.grid, sd.block, c->sell_stored, c->cgitmax,
-        c->opts.use_graph, c->pb.fuse_p_update);
+        c->opts.use_graph, c->pb.fuse_p_update, prop.name, prop.multiProcessorCount);
     if (len > 0) {
         std::strncpy(buf, tmp, (size_t)len - 1);
         buf[len - 1] = '\0';

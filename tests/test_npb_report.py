@@ -1,0 +1,51 @@
+"""The NPB-style driver: class table, operation count, argument rules (CPU),
+and a class S run with its verification line (GPU)."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "npb-gpu_amd"))
+
+
+def _mod():
+    from npbcg import npb_cg
+    return npb_cg
+
+
+def test_class_table_matches_oracle(npbcg_lib):
+    m = _mod()
+    assert m.CLASSES == oracle.NPB_CLASSES
+    assert m.NPB_SEED == oracle.NPB_SEED and m.EPSILON == oracle.NPB_EPSILON
+
+
+def test_mops_is_the_reference_operation_count(npbcg_lib):
+    m = _mod()
+    # class B: 2*75*75000*(3 + 182 + 25*(5 + 182) + 3) operations in 1 s
+    assert m.mops(75000, 13, 75, 1.0) == 2 * 75 * 75000 * (3 + 182 + 25 * 187 + 3) / 1e6
+    assert m.mops(75000, 13, 75, 0.0) == 0.0
+
+
+def test_arguments(npbcg_lib):
+    m = _mod()
+    with pytest.raises(SystemExit):
+        m.main(["--class", "S", "--n", "100"])
+    with pytest.raises(SystemExit):
+        m.main(["--n", "100"])
+
+
+@pytest.mark.gpu
+def test_class_s_report_verifies(gpu):
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "npb-gpu_amd", "npbcg", "npb_cg.py"),
+                          "--class", "S", "--json"], capture_output=True, text=True, timeout=300)
+    print(out.stdout[-2000:], out.stderr[-2000:])
+    assert out.returncode == 0
+    lines = out.stdout.splitlines()
+    assert " VERIFICATION SUCCESSFUL" in lines
+    assert " Verification    =               SUCCESSFUL" in lines
+    it_lines = [l for l in lines if l.startswith("    ") and len(l.split()) == 3]
+    assert len(it_lines) == 15 and it_lines[-1].split()[0] == "15"
