@@ -34,6 +34,7 @@ struct npbcg_batch {
     int nsys = 0, nr = 1, rows = 0;
     long long nnz = 0;
     std::vector<int> n, row_off;
+    std::vector<long long> sys_nnz;
     bool rhs_set = false;
 
     int *d_rowptr = nullptr, *d_col = nullptr;
@@ -41,6 +42,8 @@ struct npbcg_batch {
     float *d_val32 = nullptr;
     int *d_tiles = nullptr;          // tile_sys | tile_row | sys_tile
     int tiles_cap = 0;
+    int *d_block_tile = nullptr;     // workgroup -> tile of the gathering kernels
+    size_t block_tile_cap = 0;
     int *d_flags = nullptr;          // done | done_next
     BatchScalars *d_sc = nullptr;
     double *d_b = nullptr, *d_x = nullptr, *d_res = nullptr;   // [rows][nr]
@@ -76,6 +79,40 @@ int max_tiles(const npbcg_batch *b, int rt) {
 }
 
 constexpr int kMinTile = 64;
+constexpr int kXcds = 8;  // MI355X: workgroup b of a launch runs on XCD b % 8
+
+// Workgroup -> tile order of the gathering kernels.  With at least kXcds
+// systems, whole systems are placed on XCDs (greedy by nonzeros, largest
+// first) so that a system's vector is gathered from one XCD's L2; the lists
+// are interleaved to match the round-robin dispatch and padded with -1.
+// Fewer systems keep the plain order.
+std::vector<int> xcd_order(const npbcg_batch *b, const std::vector<int> &sys_tile,
+                           const std::vector<long long> &sys_nnz) {
+    const int tiles = sys_tile[b->nsys];
+    std::vector<int> out;
+    if (b->nsys < kXcds) {
+        out.resize(tiles);
+        for (int t = 0; t < tiles; t++) out[t] = t;
+        return out;
+    }
+    std::vector<int> order(b->nsys);
+    for (int s = 0; s < b->nsys; s++) order[s] = s;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return sys_nnz[x] > sys_nnz[y]; });
+    std::vector<long long> load(kXcds, 0);
+    std::vector<std::vector<int>> lists(kXcds);
+    for (int s : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += sys_nnz[s];
+        for (int t = sys_tile[s]; t < sys_tile[s + 1]; t++) lists[x].push_back(t);
+    }
+    size_t len = 0;
+    for (auto &l : lists) len = std::max(len, l.size());
+    out.assign(len * kXcds, -1);
+    for (int x = 0; x < kXcds; x++)
+        for (size_t i = 0; i < lists[x].size(); i++) out[i * kXcds + x] = lists[x][i];
+    return out;
+}
 
 int apply_shape(npbcg_batch *b, int lpr, int rt) {
     std::vector<int> ts, tr, st;
@@ -90,6 +127,19 @@ int apply_shape(npbcg_batch *b, int lpr, int rt) {
     B_HIP(hipMemcpyAsync(b->d_tiles, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice,
                          b->stream));
     B_HIP(hipStreamSynchronize(b->stream));
+    const std::vector<int> bt = xcd_order(b, st, b->sys_nnz);
+    if (bt.size() > b->block_tile_cap) {
+        (void)hipFree(b->d_block_tile);
+        b->d_block_tile = nullptr;
+        b->block_tile_cap = 0;
+        B_HIP(hipMalloc(&b->d_block_tile, sizeof(int) * bt.size()));
+        b->block_tile_cap = bt.size();
+    }
+    B_HIP(hipMemcpyAsync(b->d_block_tile, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice,
+                         b->stream));
+    B_HIP(hipStreamSynchronize(b->stream));
+    b->bd.blocks = (int)bt.size();
+    b->bd.block_tile = b->d_block_tile;
     b->bd.tiles = tiles;
     b->bd.tile_sys = b->d_tiles;
     b->bd.tile_row = b->d_tiles + tiles;
@@ -211,7 +261,7 @@ void npbcg_batch_destroy(npbcg_batch *b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (void *p : {(void *)b->d_rowptr, (void *)b->d_col, (void *)b->d_val, (void *)b->d_val32,
-                    (void *)b->d_tiles, (void *)b->d_flags, (void *)b->d_sc, (void *)b->d_b,
+                    (void *)b->d_tiles, (void *)b->d_block_tile, (void *)b->d_flags, (void *)b->d_sc, (void *)b->d_b,
                     (void *)b->d_x, (void *)b->d_res, b->d_inner, (void *)b->d_part,
                     (void *)b->d_sysout})
         (void)hipFree(p);
@@ -261,6 +311,8 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     b->n.assign(n, n + nsys);
     b->row_off.assign(nsys + 1, 0);
     for (int s = 0; s < nsys; s++) b->row_off[s + 1] = b->row_off[s] + n[s];
+    b->sys_nnz.resize(nsys);
+    for (int s = 0; s < nsys; s++) b->sys_nnz[s] = rowstr[s][n[s]];
     auto fail = [&](int rc) {
         npbcg_batch_destroy(b);
         return rc;

@@ -24,6 +24,11 @@ struct BatchDev {
     int tiles = 0;       // workgroups; tiles never cross systems
     int rows = 0;        // total rows
     int maxit = 0;       // inner steps per sweep
+    int blocks = 0;                 // workgroups of the gathering kernels (SpMM, residual)
+    const int *block_tile = nullptr;  // [blocks] tile of each of those workgroups, -1: none.
+                                      // Workgroup b runs on XCD b % 8, so the tiles of one
+                                      // system are placed on one XCD and its vector stays
+                                      // in that XCD's L2.
     const int *tile_sys = nullptr;  // [tiles]
     const int *tile_row = nullptr;  // [tiles + 1] first row of each tile
     const int *sys_tile = nullptr;  // [nsys + 1] first tile of each system

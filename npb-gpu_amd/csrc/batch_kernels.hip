@@ -109,6 +109,38 @@ __device__ __forceinline__ bool system_done(const int *__restrict__ done, int s)
     return all;
 }
 
+// acc[k] += sum over this lane's entries of row `row` of a_e * x[col_e][k]; the
+// lane takes entries sub, sub + LPR, ... with four loads in flight.
+template <typename T, int NR, int LPR>
+__device__ __forceinline__ void row_products(const BatchDev &bd, const T *__restrict__ val,
+                                             const T *__restrict__ x, int row, int sub,
+                                             T (&acc)[NR]) {
+    int e = bd.rowptr[row] + sub;
+    const int e1 = bd.rowptr[row + 1];
+    for (; e + 3 * LPR < e1; e += 4 * LPR) {
+        T a[4];
+        int c[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a[u] = val[e + u * LPR];
+            c[u] = bd.col[e + u * LPR];
+        }
+        VecT<T, NR> xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) xv[u] = load_vec<T, NR>(x + (size_t)c[u] * NR);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] = fma(a[u], xv[u].v[k], acc[k]);
+    }
+    for (; e < e1; e += LPR) {
+        const T a = val[e];
+        const VecT<T, NR> xv = load_vec<T, NR>(x + (size_t)bd.col[e] * NR);
+#pragma unroll
+        for (int k = 0; k < NR; k++) acc[k] = fma(a, xv.v[k], acc[k]);
+    }
+}
+
 }  // namespace
 
 // q = A p over one tile (LPR lanes per row), q stored, partials of p.q per RHS.
@@ -117,7 +149,8 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_kernel(BatchDev bd, const T
                                                             const T *__restrict__ p,
                                                             T *__restrict__ q,
                                                             double *__restrict__ pq_part) {
-    const int t = blockIdx.x;
+    const int t = bd.block_tile[blockIdx.x];
+    if (t < 0) return;
     const int s = bd.tile_sys[t];
     if (system_done<NR>(bd.done, s)) return;
     const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
@@ -130,13 +163,7 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_kernel(BatchDev bd, const T
         T acc[NR];
 #pragma unroll
         for (int k = 0; k < NR; k++) acc[k] = T(0);
-        const int e1 = bd.rowptr[row + 1];
-        for (int e = bd.rowptr[row] + sub; e < e1; e += LPR) {
-            const T a = val[e];
-            const VecT<T, NR> x = load_vec<T, NR>(p + (size_t)bd.col[e] * NR);
-#pragma unroll
-            for (int k = 0; k < NR; k++) acc[k] = fma(a, x.v[k], acc[k]);
-        }
+        row_products<T, NR, LPR>(bd, val, p, row, sub, acc);
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1)
 #pragma unroll
@@ -290,7 +317,8 @@ __global__ __launch_bounds__(kBlock) void batch_residual_kernel(BatchDev bd,
                                                                 const double *__restrict__ x,
                                                                 double *__restrict__ res,
                                                                 double *__restrict__ part) {
-    const int t = blockIdx.x;
+    const int t = bd.block_tile[blockIdx.x];
+    if (t < 0) return;
     const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
     const int grp = threadIdx.x / LPR, sub = threadIdx.x % LPR;
     constexpr int kGroups = kBlock / LPR;
@@ -301,13 +329,7 @@ __global__ __launch_bounds__(kBlock) void batch_residual_kernel(BatchDev bd,
         double acc[NR];
 #pragma unroll
         for (int k = 0; k < NR; k++) acc[k] = 0.0;
-        const int e1 = bd.rowptr[row + 1];
-        for (int e = bd.rowptr[row] + sub; e < e1; e += LPR) {
-            const double a = val[e];
-            const VecT<double, NR> xv = load_vec<double, NR>(x + (size_t)bd.col[e] * NR);
-#pragma unroll
-            for (int k = 0; k < NR; k++) acc[k] = fma(a, xv.v[k], acc[k]);
-        }
+        row_products<double, NR, LPR>(bd, val, x, row, sub, acc);
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1)
 #pragma unroll
@@ -399,7 +421,8 @@ hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, in
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            batch_spmm_kernel<T, NR, LPR><<<tiles, kBlock, 0, st>>>(bd, v.val, v.p, v.q, v.part_pq);
+            batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
+                                                                        v.part_pq);
             return hipGetLastError();
         });
     });
@@ -425,8 +448,8 @@ hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st)
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            batch_spmm_kernel<T, NR, LPR><<<bd.tiles, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
-                                                                       v.part_pq);
+            batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
+                                                                        v.part_pq);
             return hipGetLastError();
         });
     });
@@ -455,7 +478,7 @@ hipError_t batch_residual(const BatchDev &bd, const double *val, const double *b
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            batch_residual_kernel<NR, LPR><<<bd.tiles, kBlock, 0, st>>>(bd, val, b, x, res, part);
+            batch_residual_kernel<NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, val, b, x, res, part);
             return hipGetLastError();
         });
     });

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <mutex>
 #include <string>
@@ -127,6 +128,14 @@ struct npbcg_ctx {
         npbcg_dev::PrecondOp op;
         int loaded = -1;         // preconditioner kind whose values op holds
     } g;
+
+    // captured solve sequences, one per mode; replayed while the key matches
+    struct SolveGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        int kind = -1, maxit = 0, batch = 0;
+        double rtol = 0.0, rhs = 0.0;
+    } solve_graph[2];
 
     // residual/correction pairs (lazy)
     struct Pairs {
@@ -547,6 +556,10 @@ void npbcg_destroy(npbcg_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
+    for (auto &sg : c->solve_graph) {
+        if (sg.ex) (void)hipGraphExecDestroy(sg.ex);
+        if (sg.g) (void)hipGraphDestroy(sg.g);
+    }
     for (auto e : c->probe_start) (void)hipEventDestroy(e);
     for (auto e : c->probe_stop) (void)hipEventDestroy(e);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
@@ -1300,6 +1313,50 @@ int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stat
 
 namespace {
 
+// Runs the first stretch of a solve: replays its captured graph when the
+// key (mode, preconditioner, rtol, maxit, batch, ||b||) matches the last
+// capture, else captures `enqueue` anew (or, with use_graph = 0, launches it
+// directly).
+int run_solve_sequence(npbcg_ctx *c, int mode, double rtol, int maxit, int batch,
+                       const std::function<hipError_t()> &enqueue) {
+    if (!c->opts.use_graph) {
+        NPBCG_HIP(enqueue());
+        return NPBCG_OK;
+    }
+    auto &sg = c->solve_graph[mode];
+    if (!(sg.ex && sg.kind == c->precond && sg.rtol == rtol && sg.maxit == maxit &&
+          sg.batch == batch && sg.rhs == c->rhs_norm)) {
+        if (sg.ex) (void)hipGraphExecDestroy(sg.ex);
+        if (sg.g) (void)hipGraphDestroy(sg.g);
+        sg = npbcg_ctx::SolveGraph{};
+        NPBCG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue();
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return set_error(NPBCG_ERR_DEVICE, std::string("solve graph capture failed: ") +
+                                                   hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        hipGraphExec_t ex = nullptr;
+        hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return set_error(NPBCG_ERR_DEVICE, std::string("hipGraphInstantiate: ") +
+                                                   hipGetErrorString(e3));
+        }
+        sg.g = g;
+        sg.ex = ex;
+        sg.kind = c->precond;
+        sg.rtol = rtol;
+        sg.maxit = maxit;
+        sg.batch = batch;
+        sg.rhs = c->rhs_norm;
+    }
+    NPBCG_HIP(hipGraphLaunch(sg.ex, c->stream));
+    return NPBCG_OK;
+}
+
 int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats *st) {
     if (!c || !(rtol > 0.0) || maxit < 1 ||
         (mode != NPBCG_SOLVE_FP64 && mode != NPBCG_SOLVE_MIXED))
@@ -1329,14 +1386,24 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
     Scalars sc{};
     double relres = 1.0;
     if (mode == NPBCG_SOLVE_FP64) {
-        NPBCG_HIP(npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 1));
-        NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
-        NPBCG_HIP(npbcg_dev::launch_cg_init(pbs, 0, c->stream));
         int batch = std::min(batch0, maxit);
+        // first batch: reset, init, `batch` steps, residual (captured once per key)
+        auto first_batch = [&]() -> hipError_t {
+            hipError_t e = npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 1);
+            if (e == hipSuccess) e = npbcg_dev::launch_cg_init(pbs, 0, c->stream);
+            if (e == hipSuccess) e = npbcg_dev::enqueue_solve_steps(pbs, batch, 1, c->stream);
+            if (e == hipSuccess) e = npbcg_dev::launch_resid(pbs, 0, c->stream);
+            return e;
+        };
+        NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
+        int rc = run_solve_sequence(c, 0, rtol, maxit, batch, first_batch);
+        if (rc != NPBCG_OK) return rc;
         bool first = true;
         for (;;) {
-            NPBCG_HIP(npbcg_dev::enqueue_solve_steps(pbs, batch, first ? 1 : 0, c->stream));
-            NPBCG_HIP(npbcg_dev::launch_resid(pbs, 0, c->stream));
+            if (!first) {
+                NPBCG_HIP(npbcg_dev::enqueue_solve_steps(pbs, batch, 0, c->stream));
+                NPBCG_HIP(npbcg_dev::launch_resid(pbs, 0, c->stream));
+            }
             NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
             int rc = read_scalars_from(c, pbs.sc, &sc);
             if (rc != NPBCG_OK) return rc;
@@ -1355,36 +1422,51 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
         const npbcg_dev::MixedProblem &mp = c->mp;
         Fp64Problem inner = pbs;  // same launch shapes, inner scalars
         inner.sc = mp.sc;
-        NPBCG_HIP(npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 0));
+        // Sweeps are decided on the device (launch_sweep_prepare), so two are
+        // enqueued before the first host synchronisation (FP32 holds about
+        // 1e-6 per sweep, so two reach 1e-10 from x = 0 on well-conditioned
+        // systems); later ones go one at a time.  maxit is checked per sweep.
+        auto sweep = [&](int index) -> hipError_t {
+            const int steps = index == 0 ? batch0 - 2 : batch0 / 2 + 1;
+            hipError_t e = npbcg_dev::launch_sweep_prepare(pbs.sc, mp.sc, c->rhs_norm, rtol,
+                                                           maxit, index == 0 ? 1 : 0, c->stream);
+            if (e == hipSuccess)
+                e = npbcg_dev::enqueue_inner32(pbs, mp, index == 0 ? 1 : 0, steps, c->stream);
+            if (e == hipSuccess) e = npbcg_dev::launch_refine(pbs, mp, c->stream);
+            if (e == hipSuccess) e = npbcg_dev::launch_resid(pbs, 0, c->stream);
+            return e;
+        };
+        auto first_two = [&]() -> hipError_t {
+            hipError_t e = npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 0);
+            if (e == hipSuccess) e = npbcg_dev::launch_reset_scalars(inner, 0.0, c->stream, 1e-6, 1);
+            if (e == hipSuccess)
+                e = npbcg_dev::launch_fill(pbs.z, c->n + 1, 0.0, c->pb.vec_blocks, c->stream);
+            if (e == hipSuccess) e = sweep(0);
+            if (e == hipSuccess) e = sweep(1);
+            return e;
+        };
         NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
-        NPBCG_HIP(npbcg_dev::launch_fill(pbs.z, c->n + 1, 0.0, c->pb.vec_blocks, c->stream));
-        int inner_steps = 0;
-        for (int sweep = 0;; sweep++) {
-            // Inner tolerance relative to this sweep's residual: FP32 holds
-            // about 1e-6 of it; ask for no more than the outer target needs.
-            const double tol_in = std::max(1e-6, 0.5 * rtol / relres);
-            const int steps = std::min(sweep == 0 ? batch0 - 2 : batch0 / 2 + 1,
-                                       maxit - inner_steps);
-            NPBCG_HIP(npbcg_dev::launch_reset_scalars(inner, 0.0, c->stream, tol_in, 1));
-            NPBCG_HIP(npbcg_dev::enqueue_inner32(pbs, mp, sweep == 0 ? 1 : 0, steps,
-                                                 c->stream));
-            NPBCG_HIP(npbcg_dev::launch_refine(pbs, mp, c->stream));
-            NPBCG_HIP(npbcg_dev::launch_resid(pbs, 0, c->stream));
+        int rc0 = run_solve_sequence(c, 1, rtol, maxit, batch0, first_two);
+        if (rc0 != NPBCG_OK) return rc0;
+        int enqueued = 2, inner_steps = 0;
+        for (bool first = true;; first = false) {
+            if (!first) NPBCG_HIP(sweep(enqueued++));
             NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
             Scalars si{};
             int rc = read_scalars_from(c, mp.sc, &si);
             if (rc != NPBCG_OK) return rc;
             rc = read_scalars_from(c, pbs.sc, &sc);
             if (rc != NPBCG_OK) return rc;
-            inner_steps += si.steps;
-            s.sweeps++;
+            inner_steps = sc.inner_total + si.steps;
             relres = sc.rnorm / c->rhs_norm;
-            if (relres <= rtol || inner_steps >= maxit || steps <= 0) break;
+            if (relres <= rtol || inner_steps >= maxit || enqueued >= 1000) break;
         }
+        s.sweeps = sc.sweeps;
         s.iterations = inner_steps;
         s.fp32_spmvs = inner_steps;
         s.fp64_spmvs = s.sweeps;
     }
+    NPBCG_HIP(npbcg_dev::launch_clear_done(pbs, c->stream, 1));
     NPBCG_HIP(hipEventElapsedTime(&ms_total, c->ev_start, c->ev_stop));
     s.relres = relres;
     s.time_ms = ms_total;

@@ -66,6 +66,10 @@ struct Scalars {
     int stop_enabled;  // solve mode: CG steps stop once r.r <= tol2
     int done;        // set when the stop test passed; later CG steps return at once
     int steps;       // CG steps executed since the last reset
+    int skip_resid;  // mixed solve: the residual pass of a sweep that is not needed
+    int sweeps;      // mixed solve: refinement sweeps run
+    int inner_total; // mixed solve: inner steps of the finished sweeps
+    int pad;
 };
 
 // Sum of this workgroup's per-thread values, written to part[blockIdx.x]

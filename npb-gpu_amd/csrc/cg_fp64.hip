@@ -66,6 +66,7 @@ __global__ __launch_bounds__(kBlock) void spmv_csr_kernel(
     double *__restrict__ y, const double *__restrict__ x, int n,
     double *__restrict__ part, const Scalars *__restrict__ sc) {
     if (MODE == 0 && sc->done) return;  // a solve already met its tolerance
+    if (MODE == 1 && sc->skip_resid) return;  // a refinement sweep that is not needed
     constexpr int kGroups = kBlock / SEG;
     const int sl = threadIdx.x % SEG;
     // With a whole wavefront per row the row index is wave-uniform; saying so
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
     const Scalars *__restrict__ sc) {
     if (MODE == 0 && sc->done) return;  // a solve already met its tolerance
+    if (MODE == 1 && sc->skip_resid) return;  // a refinement sweep that is not needed
     // Dynamic LDS: W vector entries, then one p.q slot per slice of this
     // workgroup (the p.q partial is summed in slice order).
     extern __shared__ double2 lds_v2[];  // 16-byte aligned
@@ -233,7 +235,9 @@ __global__ __launch_bounds__(kBlock) void combine_rows_kernel(
 // ||x - r||^2, x.z and z.z (stride gridDim.x).
 __global__ __launch_bounds__(kBlock) void resid_combine_kernel(
     const double *__restrict__ rows_part, int nblk, const double *__restrict__ x,
-    const double *__restrict__ z, double *__restrict__ r, int n, double *__restrict__ res) {
+    const double *__restrict__ z, double *__restrict__ r, int n, double *__restrict__ res,
+    const Scalars *__restrict__ sc) {
+    if (sc->skip_resid) return;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         double rj = rows_part[j];
@@ -366,6 +370,7 @@ __global__ __launch_bounds__(kBlock) void resid_finish_kernel(
     const double *__restrict__ res, int count, Scalars *sc,
     double *__restrict__ zeta_hist, double *__restrict__ rnorm_hist, int hist_cap,
     int record) {
+    if (sc->skip_resid) return;
     const double s0 = reduce_partials<kBlock>(res, count);
     const double s1 = reduce_partials<kBlock>(res + count, count);
     const double s2 = reduce_partials<kBlock>(res + 2 * count, count);
@@ -442,12 +447,21 @@ __global__ void reset_scalars_kernel(Scalars *sc, double shift, double rtol2, in
         sc->stop_enabled = stop;
         sc->done = 0;
         sc->steps = 0;
+        sc->skip_resid = 0;
+        sc->sweeps = 0;
+        sc->inner_total = 0;
     }
 }
 
 // Clears the stop flag (a solve restarting from its true residual).
-__global__ void clear_done_kernel(Scalars *sc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) sc->done = 0;
+__global__ void clear_done_kernel(Scalars *sc, int end_of_solve) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->done = 0;
+        if (end_of_solve) {  // back to conj_grad / power use of the same scalars
+            sc->stop_enabled = 0;
+            sc->skip_resid = 0;
+        }
+    }
 }
 
 __global__ void fill_kernel(double *__restrict__ v, int count, double value) {
@@ -591,7 +605,7 @@ hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
     if (e != hipSuccess) return e;
     if (pb.spmv_variant == kSpmvTiled) {
         resid_combine_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
-            pb.sell.rows_part, pb.sell.nblk, pb.x, pb.z, pb.r, pb.n, pb.part.res);
+            pb.sell.rows_part, pb.sell.nblk, pb.x, pb.z, pb.r, pb.n, pb.part.res, pb.sc);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -658,8 +672,8 @@ hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_clear_done(const Fp64Problem &pb, hipStream_t st) {
-    clear_done_kernel<<<1, 64, 0, st>>>(pb.sc);
+hipError_t launch_clear_done(const Fp64Problem &pb, hipStream_t st, int end_of_solve) {
+    clear_done_kernel<<<1, 64, 0, st>>>(pb.sc, end_of_solve);
     return hipGetLastError();
 }
 

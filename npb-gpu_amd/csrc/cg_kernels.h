@@ -109,7 +109,9 @@ hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_scalars,
 // relative-residual stop test r.r <= rtol^2 * r0.r0 inside the CG steps.
 hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st,
                                 double rtol = 0.0, int stop = 0);
-hipError_t launch_clear_done(const Fp64Problem &pb, hipStream_t st);
+// Clears the stop flag; end_of_solve also returns the scalars to conj_grad /
+// power use (no stop test, no skipped residual passes).
+hipError_t launch_clear_done(const Fp64Problem &pb, hipStream_t st, int end_of_solve = 0);
 hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream_t st);
 // Copy of bytes (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
@@ -156,6 +158,14 @@ hipError_t launch_residual_restart(const Fp64Problem &pb, hipStream_t st);
 hipError_t launch_narrow(const double *in, float *out, int n, int blocks, hipStream_t st);
 // minv = 1/diag(A) from the CSR.
 hipError_t launch_jacobi(const Fp64Problem &pb, double *minv, hipStream_t st);
+
+// Start of a refinement sweep of the mixed solve, decided on the device from
+// the FP64 true residual of the previous sweep (outer->rnorm): if it met rtol
+// (or maxit inner steps are used up) the inner solve and this sweep's residual
+// pass are skipped; else the inner stop level max(1e-6, rtol/(2 relres))^2 is
+// set.  Lets several sweeps be enqueued with one host synchronisation.
+hipError_t launch_sweep_prepare(Scalars *outer, Scalars *inner, double rhs_norm, double rtol,
+                                int maxit, int first, hipStream_t st);
 
 size_t scalars_bytes();
 

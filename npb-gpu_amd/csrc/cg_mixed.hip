@@ -176,6 +176,28 @@ __global__ __launch_bounds__(kBlock) void update_p32_kernel(
     }
 }
 
+__global__ void sweep_prepare_kernel(Scalars *outer, Scalars *inner, double rhs_norm,
+                                     double rtol, int maxit, int first) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (first) {
+            outer->inner_total = 0;
+            outer->sweeps = 0;
+        } else {
+            outer->inner_total += inner->steps;
+        }
+        const double rel = first ? 1.0 : outer->rnorm / rhs_norm;
+        const bool stop = (!first && rel <= rtol) || outer->inner_total >= maxit;
+        const double t = fmax(1e-6, 0.5 * rtol / fmax(rel, 1e-300));
+        inner->rtol2 = t * t;
+        inner->tol2 = 0.0;
+        inner->stop_enabled = 1;
+        inner->steps = 0;
+        inner->done = stop ? 1 : 0;
+        outer->skip_resid = stop ? 1 : 0;
+        if (!stop) outer->sweeps += 1;
+    }
+}
+
 // x += d (FP64 accumulation of the FP32 correction).
 __global__ __launch_bounds__(kBlock) void refine_kernel(double *__restrict__ x,
                                                        const float *__restrict__ d, int n) {
@@ -283,6 +305,12 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
         e = hipGetLastError();
     }
     return e;
+}
+
+hipError_t launch_sweep_prepare(Scalars *outer, Scalars *inner, double rhs_norm, double rtol,
+                                int maxit, int first, hipStream_t st) {
+    sweep_prepare_kernel<<<1, 64, 0, st>>>(outer, inner, rhs_norm, rtol, maxit, first);
+    return hipGetLastError();
 }
 
 hipError_t launch_refine(const Fp64Problem &pb, const MixedProblem &mp, hipStream_t st) {

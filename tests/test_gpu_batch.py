@@ -90,3 +90,19 @@ def test_batch_reports_unconverged_and_errors(gpu):
         assert st.converged < st.solves and np.all(steps <= 3)
     with pytest.raises(gpu.NpbcgError):
         gpu.Batch(mats, nrhs=3)
+
+
+def test_batch_many_systems_xcd_placement(gpu):
+    # 12 systems: tiles are placed per XCD (at least 8 systems), uneven sizes
+    mats = _systems(gpu, 12, seed=21)
+    rhs = _rhs(mats, 4)
+    with gpu.Batch(mats, nrhs=4, mode=1) as bt:
+        bt.set_rhs(rhs)
+        st, steps, rel = bt.solve(RTOL, 3000)
+        xs = bt.solution()
+    assert st.converged == 48 and st.max_relres <= RTOL
+    for s, m in enumerate(mats):
+        om = _ocsr(m)
+        for k in range(4):
+            b = rhs[s][:, k]
+            assert np.linalg.norm(b - oracle.spmv(om, xs[s][:, k])) <= RTOL * np.linalg.norm(b)

@@ -128,3 +128,20 @@ def test_solve_jacobi_pcg(gpu, mode):
     assert err <= 1e-8
     if mode == 0:   # same recurrence as the oracle's PCG: step counts agree closely
         assert abs(st.iterations - oit) <= 2
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_conj_grad_after_solve_is_unchanged(gpu, golden, mode):
+    # A solve leaves its stop flag behind on the device; the drop-in conj_grad
+    # on the same context must not see it.
+    c = golden["S_npb"]
+    m = gpu.makea(c["n"], c["nonzer"], c["shift"], c["seed"])
+    x = np.ones(m.n + 2)
+    with gpu.Context(m, spmv_variant=5) as ctx:
+        before = ctx.conj_grad(x)
+        ctx.set_rhs(oracle.make_rhs(m.n, 3.0))
+        ctx.solve(mode, RTOL, 500)
+        after = ctx.conj_grad(x)
+    for u, v in zip(before[:4], after[:4]):
+        assert np.array_equal(u, v)
+    assert before[4] == after[4]

@@ -1,6 +1,7 @@
 """The NPB-style driver: class table, operation count, argument rules (CPU),
 and a class S run with its verification line (GPU)."""
 import os
+import re
 import subprocess
 import sys
 
@@ -47,5 +48,5 @@ def test_class_s_report_verifies(gpu):
     lines = out.stdout.splitlines()
     assert " VERIFICATION SUCCESSFUL" in lines
     assert " Verification    =               SUCCESSFUL" in lines
-    it_lines = [l for l in lines if l.startswith("    ") and len(l.split()) == 3]
+    it_lines = [l for l in lines if re.fullmatch(r" {4,}\d+ +\S+e[-+]\d+ *\S+e[-+]\d+", l)]
     assert len(it_lines) == 15 and it_lines[-1].split()[0] == "15"
