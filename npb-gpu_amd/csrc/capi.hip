@@ -109,7 +109,8 @@ struct npbcg_ctx {
 
     // top-K graph and the sparse preconditioner operator (lazy)
     struct Graph {
-        bool built = false;
+        bool graph_built = false;  // device graph (ensure_graph_device)
+        bool built = false;        // plus the operator (ensure_row_graph)
         double *adiag = nullptr;
         float *feat = nullptr;
         int *nbr = nullptr;
@@ -943,9 +944,11 @@ int ensure_precond_buffers(npbcg_ctx *c) {
 // The top-K graph (device) and the CSR layout of the sparse operator M^-1:
 // row j holds its diagonal, its K out-neighbours and its in-neighbours (rows
 // whose top-K contain j).
-int ensure_row_graph(npbcg_ctx *c) {
+// The top-K graph on the device (features, neighbours, in-edge lists) and the
+// zero (g, h) of the Neumann form: all the network and its loss need.
+int ensure_graph_device(npbcg_ctx *c) {
     auto &g = c->g;
-    if (g.built) return NPBCG_OK;
+    if (g.graph_built) return NPBCG_OK;
     const int n = c->n;
     const int K = npbcg_dev::kNbr;
     const size_t nk = (size_t)n * K;
@@ -963,6 +966,21 @@ int ensure_row_graph(npbcg_ctx *c) {
                                                g.feat, g.nbr, g.wgt, g.nval, c->stream));
     NPBCG_HIP(npbcg_dev::launch_transpose(g.nbr, g.wgt, g.nval, n, g.cnt, g.tptr, g.tsrc, g.tw,
                                           g.tval, c->stream));
+    NPBCG_HIP(hipMalloc(&g.gh_zero, sizeof(float) * 2 * n));
+    NPBCG_HIP(hipMemsetAsync(g.gh_zero, 0, sizeof(float) * 2 * n, c->stream));
+    g.graph_built = true;
+    return NPBCG_OK;
+}
+
+// The CSR operator of M^-1 (structure built on the host from the graph).
+int ensure_row_graph(npbcg_ctx *c) {
+    auto &g = c->g;
+    if (g.built) return NPBCG_OK;
+    int rc = ensure_graph_device(c);
+    if (rc != NPBCG_OK) return rc;
+    const int n = c->n;
+    const int K = npbcg_dev::kNbr;
+    const size_t nk = (size_t)n * K;
     std::vector<int> nbr(nk), tptr(n + 1);
     std::vector<float> nval(nk);
     NPBCG_HIP(hipMemcpyAsync(nbr.data(), g.nbr, sizeof(int) * nk, hipMemcpyDeviceToHost,
@@ -1005,7 +1023,6 @@ int ensure_row_graph(npbcg_ctx *c) {
     NPBCG_HIP(hipMalloc(&g.op_src, sizeof(float) * slots));
     NPBCG_HIP(hipMalloc(&g.op_val, sizeof(double) * slots));
     NPBCG_HIP(hipMalloc(&g.op_val32, sizeof(float) * slots));
-    NPBCG_HIP(hipMalloc(&g.gh_zero, sizeof(float) * 2 * n));
     NPBCG_HIP(hipMalloc(&g.gh_learned, sizeof(float) * 2 * n));
     NPBCG_HIP(hipMalloc(&g.pz, sizeof(double) * c->vstride));
     NPBCG_HIP(hipMalloc(&g.pz32, sizeof(float) * c->vstride));
@@ -1015,7 +1032,6 @@ int ensure_row_graph(npbcg_ctx *c) {
                              c->stream));
     NPBCG_HIP(hipMemcpyAsync(g.op_src, src.data(), sizeof(float) * slots, hipMemcpyHostToDevice,
                              c->stream));
-    NPBCG_HIP(hipMemsetAsync(g.gh_zero, 0, sizeof(float) * 2 * n, c->stream));
     NPBCG_HIP(hipMemsetAsync(g.gh_learned, 0, sizeof(float) * 2 * n, c->stream));
     NPBCG_HIP(hipMemsetAsync(g.pz, 0, sizeof(double) * c->vstride, c->stream));
     NPBCG_HIP(hipMemsetAsync(g.pz32, 0, sizeof(float) * c->vstride, c->stream));
@@ -1084,7 +1100,7 @@ namespace npbcg_detail {
 
 // The context's matrix as the network sees it (builds the graph if needed).
 int ctx_net_system(npbcg_ctx *c, npbcg_dev::NetSystem *sys) {
-    int rc = ensure_row_graph(c);
+    int rc = ensure_graph_device(c);
     if (rc != NPBCG_OK) return rc;
     const auto &g = c->g;
     sys->n = c->n;
@@ -1272,7 +1288,7 @@ int npbcg_set_pairs(npbcg_ctx *c, int pairs, const float *r, const float *e) {
 int npbcg_graph(npbcg_ctx *c, int *nbr, float *wgt, float *nval, float *feat) {
     if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_graph: NULL context");
     NPBCG_HIP(hipSetDevice(c->device));
-    int rc = ensure_row_graph(c);
+    int rc = ensure_graph_device(c);
     if (rc != NPBCG_OK) return rc;
     const size_t nk = (size_t)c->n * npbcg_dev::kNbr;
     const auto &g = c->g;

@@ -31,29 +31,39 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 namespace {
 
+// Workgroups of a grid-stride kernel (capped; the kernel loops).
 int blocks_for(long long items, int per_block = kBlock, int cap = 65535) {
     long long b = (items + per_block - 1) / per_block;
     return (int)std::max<long long>(1, std::min<long long>(b, cap));
 }
 
+// Workgroups of a kernel that covers exactly one item per thread group (no
+// loop, so no cap).
+int exact_blocks(long long items, int per_block) {
+    return (int)std::max<long long>(1, (items + per_block - 1) / per_block);
+}
+
 }  // namespace
 
-int row_blocks(int n) { return blocks_for((long long)n * 16); }
+int row_blocks(int n) { return exact_blocks((long long)n * 16, kBlock); }
 
 // ---------------------------------------------------------------------------
 // Graph and features (once per matrix)
 // ---------------------------------------------------------------------------
 
+// a_jj of every row, 16 lanes per row (0 when the row stores no diagonal).
 __global__ __launch_bounds__(kBlock) void diag_kernel(const int *__restrict__ rowstr,
                                                      const int *__restrict__ colidx,
                                                      const double *__restrict__ a,
                                                      double *__restrict__ adiag, int n) {
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        double d = 0.0;
-        for (int k = rowstr[j]; k < rowstr[j + 1]; k++)
+    const long long gid = blockIdx.x * (long long)kBlock + threadIdx.x;
+    const int j = (int)(gid / 16), sub = (int)(gid % 16);
+    double d = 0.0;
+    if (j < n)
+        for (int k = rowstr[j] + sub; k < rowstr[j + 1]; k += 16)
             if (colidx[k] == j) d = a[k];
-        adiag[j] = d;
-    }
+    d = segment_sum<16>(d);  // one lane holds the diagonal, the others 0
+    if (j < n && sub == 0) adiag[j] = d;
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -177,28 +187,28 @@ __global__ __launch_bounds__(kBlock) void indegree_kernel(const int *__restrict_
         if (wgt[e] != 0.0f) atomicAdd(&cnt[nbr[e]], 1);
 }
 
-// Exclusive prefix sum of cnt[0..n) into ptr[0..n] (one 1024-thread block).
+// Exclusive prefix sum of cnt[0..n) into ptr[0..n] (one 1024-thread block):
+// per-thread chunk sums, a Hillis-Steele scan of the 1024 chunk sums in LDS,
+// then each thread writes its chunk.
 __global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ cnt,
                                                     int *__restrict__ ptr, int n) {
-    __shared__ int part[1024];
+    __shared__ int part[2][1024];
     const int t = threadIdx.x;
     const int chunk = (n + 1023) / 1024;
     const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
     int s = 0;
     for (int i = lo; i < hi; i++) s += cnt[i];
-    part[t] = s;
+    part[0][t] = s;
     __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int i = 0; i < 1024; i++) {
-            const int v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
-        ptr[n] = acc;
+    int cur = 0;
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = part[cur][t] + (t >= off ? part[cur][t - off] : 0);
+        part[cur ^ 1][t] = v;
+        cur ^= 1;
+        __syncthreads();
     }
-    __syncthreads();
-    int acc = part[t];
+    int acc = part[cur][t] - s;  // exclusive
+    if (t == 1023) ptr[n] = part[cur][t];
     for (int i = lo; i < hi; i++) {
         ptr[i] = acc;
         acc += cnt[i];
@@ -826,7 +836,7 @@ __global__ __launch_bounds__(kApplyBlock) void precond_apply_kernel(
 hipError_t launch_graph_features(const int *rowstr, const int *colidx, const double *a, int n,
                                  double *adiag, float *feat, int *nbr, float *wgt, float *nval,
                                  hipStream_t st) {
-    diag_kernel<<<blocks_for(n), kBlock, 0, st>>>(rowstr, colidx, a, adiag, n);
+    diag_kernel<<<row_blocks(n), kBlock, 0, st>>>(rowstr, colidx, a, adiag, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     graph_features_kernel<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(
@@ -885,7 +895,7 @@ hipError_t launch_net_forward(const NetBuffers &nb, const NetSystem &sys, const 
     build_x0_kernel<<<blocks_for((long long)n * kK0), kBlock, 0, st>>>(sys.feat, nb.norm, n,
                                                                       nb.x0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int gb = blocks_for(n, 16 * kWavesPerBlock);
+    const int gb = exact_blocks(n, 16 * kWavesPerBlock);
     gemm_fwd_kernel<kK0><<<gb, kBlock, 0, st>>>(nb.x0, kK0, n, wb + kWtTransposed,
                                                 params + kOffB0, nb.x[0], kXW);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -923,7 +933,7 @@ hipError_t launch_net_backward(const NetBuffers &nb, const NetSystem &sys, const
     if ((e = hipGetLastError()) != hipSuccess) return e;
     reduce_parts_kernel<<<1, kBlock, 0, st>>>(nb.rpart, rb, kReadout, kReadout, grad + kOffR);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int gb = blocks_for(n, 16 * kWavesPerBlock);
+    const int gb = exact_blocks(n, 16 * kWavesPerBlock);
     const int wgb = std::min(nb.wparts, blocks_for(n, 32));
     for (int l = kLayers - 1; l >= 0; l--) {
         gemm_bwd_weight_kernel<kXW><<<wgb, kBlock, 0, st>>>(nb.x[l], kXW, n, nb.dz, nb.wpart);
