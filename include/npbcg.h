@@ -326,7 +326,9 @@ typedef struct npbcg_batch_opts {
     int lanes_per_row;   /* SpMM lanes per row: 4, 8, 16, 32, 64; 0 = tuned or default */
     int rows_per_tile;   /* rows per workgroup tile: 64 .. 4096; 0 = tuned or default */
     int inner_steps;     /* MIXED: inner CG steps per sweep at most (0: 60) */
-    int reserved[10];
+    int layout;          /* SpMM: 0 = tuned or column blocks, 1 = row gathers,
+                            2 = column blocks (vector blocks staged in LDS) */
+    int reserved[9];
 } npbcg_batch_opts;
 
 typedef struct npbcg_batch_stats {
@@ -336,7 +338,8 @@ typedef struct npbcg_batch_stats {
     int max_steps;       /* largest inner CG step count of a solve */
     int lanes_per_row;
     int rows_per_tile;
-    int reserved_i[2];
+    int layout;          /* 1 row gathers, 2 column blocks */
+    int reserved_i;
     double time_ms;      /* HIP events around the device work */
     double max_relres;
     double reserved_d[4];
@@ -356,7 +359,7 @@ int npbcg_batch_set_rhs(npbcg_batch *batch, const double *b);
 int npbcg_batch_solve(npbcg_batch *batch, double rtol, int maxit, npbcg_batch_stats *stats,
                       int *steps, double *relres);
 int npbcg_batch_get_solution(npbcg_batch *batch, double *x);
-/* Autotuner: times the SpMM of this batch for every (lanes_per_row,
+/* Autotuner: times the SpMM of this batch for every (layout, lanes_per_row,
  * rows_per_tile) candidate, keeps the fastest, and records it in the text
  * file cache_path (may be NULL) under the batch's shape; a later call with the
  * same shape and file reuses the record without timing.  ms (may be NULL)

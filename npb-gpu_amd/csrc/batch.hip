@@ -44,11 +44,21 @@ struct npbcg_batch {
     int tiles_cap = 0;
     int *d_block_tile = nullptr;     // workgroup -> tile of the gathering kernels
     size_t block_tile_cap = 0;
+    // column-blocked SpMM
+    int cbw = 0;                     // column block width
+    std::vector<int> sys_nb;         // blocks per system
+    int *d_items = nullptr;          // [items][4]
+    int items_cap = 0;
+    int *d_sysinfo = nullptr;        // sys_item | sys_row | sys_nb
+    long long *d_sys64 = nullptr;    // sys_seg | sys_part
+    int *d_seg = nullptr;            // per-row block offsets
+    void *d_rows_part = nullptr;     // per-block row partials (inner type)
     int *d_flags = nullptr;          // done | done_next
     BatchScalars *d_sc = nullptr;
     double *d_b = nullptr, *d_x = nullptr, *d_res = nullptr;   // [rows][nr]
     void *d_inner = nullptr;         // d, r, p, q in the inner type
-    double *d_part = nullptr;        // 4 * tiles_cap * nr (init, pq, rr0, rr1)
+    double *d_part = nullptr;        // 4 * part_cap * nr (init, pq, rr0, rr1)
+    int part_cap = 0;                // max(tiles, items) at the smallest tile
     double *d_sysout = nullptr;      // nsys * nr
     BatchDev bd;
 };
@@ -114,9 +124,33 @@ std::vector<int> xcd_order(const npbcg_batch *b, const std::vector<int> &sys_til
     return out;
 }
 
-int apply_shape(npbcg_batch *b, int lpr, int rt) {
+int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
     std::vector<int> ts, tr, st;
     make_tiles(b, rt, ts, tr, st);
+    if (layout == 2) {  // items: (system, column block, row range) in system order
+        std::vector<int> items, sys_item(b->nsys + 1, 0);
+        for (int s = 0; s < b->nsys; s++) {
+            sys_item[s] = (int)(items.size() / 4);
+            for (int blk = 0; blk < b->sys_nb[s]; blk++)
+                for (int r = 0; r < b->n[s]; r += rt) {
+                    items.push_back(s);
+                    items.push_back(blk);
+                    items.push_back(b->row_off[s] + r);
+                    items.push_back(b->row_off[s] + std::min(b->n[s], r + rt));
+                }
+        }
+        sys_item[b->nsys] = (int)(items.size() / 4);
+        const int ni = sys_item[b->nsys];
+        if (ni > b->items_cap) return set_error(NPBCG_ERR_INTERNAL, "batch: item capacity");
+        B_HIP(hipMemcpyAsync(b->d_items, items.data(), sizeof(int) * items.size(),
+                             hipMemcpyHostToDevice, b->stream));
+        B_HIP(hipMemcpyAsync(b->d_sysinfo, sys_item.data(), sizeof(int) * (b->nsys + 1),
+                             hipMemcpyHostToDevice, b->stream));
+        B_HIP(hipStreamSynchronize(b->stream));
+        b->bd.items = ni;
+    }
+    b->bd.cb = layout == 2 ? 1 : 0;
+    b->opts.layout = layout;
     const int tiles = (int)ts.size();
     if (tiles > b->tiles_cap) return set_error(NPBCG_ERR_INTERNAL, "batch: tile capacity");
     std::vector<int> all;
@@ -150,6 +184,19 @@ int apply_shape(npbcg_batch *b, int lpr, int rt) {
     return NPBCG_OK;
 }
 
+// Sizes of the column-blocked layout for the batch's inner type: blocks of
+// cbw columns whose nrhs-wide vector entries fill lds bytes.
+int cb_width(const npbcg_batch *b, int device) {
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) !=
+            hipSuccess || optin <= 0)
+        optin = 64 * 1024;
+    const int elem = b->opts.mode == NPBCG_SOLVE_MIXED ? 4 : 8;
+    const int want = b->opts.mode == NPBCG_SOLVE_MIXED ? 64 * 1024 : 128 * 1024;
+    const int lds = std::min(want, optin - 2048);
+    return std::max(64, lds / (elem * b->nr));
+}
+
 template <typename T>
 BatchVecs<T> inner_vecs(const npbcg_batch *b) {
     BatchVecs<T> v;
@@ -161,7 +208,8 @@ BatchVecs<T> inner_vecs(const npbcg_batch *b) {
     v.r = base + m;
     v.p = base + 2 * m;
     v.q = base + 3 * m;
-    const size_t pt = (size_t)b->tiles_cap * b->nr;
+    const size_t pt = (size_t)b->part_cap * b->nr;
+    v.rows_part = static_cast<T *>(b->d_rows_part);
     v.part_init = b->d_part;
     v.part_pq = b->d_part + pt;
     v.part_rr[0] = b->d_part + 2 * pt;
@@ -176,18 +224,22 @@ std::string shape_key(const npbcg_batch *b) {
     return buf;
 }
 
-bool cache_lookup(const char *path, const std::string &key, int *lpr, int *rt, double *ms) {
+// Record file: one line per tuning, "<shape key> <lanes> <rows> <layout> <ms>";
+// the last record of a key wins.
+bool cache_lookup(const char *path, const std::string &key, int *lpr, int *rt, int *layout,
+                  double *ms) {
     if (!path) return false;
     FILE *f = std::fopen(path, "r");
     if (!f) return false;
     char k[512];
-    int a = 0, c = 0;
+    int a = 0, c = 0, l = 0;
     double t = 0;
     bool found = false;
-    while (std::fscanf(f, "%511s %d %d %lf", k, &a, &c, &t) == 4)
-        if (key == k) {  // the last record of a key wins
+    while (std::fscanf(f, "%511s %d %d %d %lf", k, &a, &c, &l, &t) == 5)
+        if (key == k) {
             *lpr = a;
             *rt = c;
+            *layout = l;
             *ms = t;
             found = true;
         }
@@ -195,11 +247,12 @@ bool cache_lookup(const char *path, const std::string &key, int *lpr, int *rt, d
     return found;
 }
 
-void cache_store(const char *path, const std::string &key, int lpr, int rt, double ms) {
+void cache_store(const char *path, const std::string &key, int lpr, int rt, int layout,
+                 double ms) {
     if (!path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %.6f\n", key.c_str(), lpr, rt, ms);
+    std::fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), lpr, rt, layout, ms);
     std::fclose(f);
 }
 
@@ -260,7 +313,9 @@ void npbcg_batch_destroy(npbcg_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    for (void *p : {(void *)b->d_rowptr, (void *)b->d_col, (void *)b->d_val, (void *)b->d_val32,
+    for (void *p : {(void *)b->d_items, (void *)b->d_sysinfo, (void *)b->d_sys64, (void *)b->d_seg,
+                    b->d_rows_part, (void *)b->d_rowptr, (void *)b->d_col, (void *)b->d_val,
+                    (void *)b->d_val32,
                     (void *)b->d_tiles, (void *)b->d_block_tile, (void *)b->d_flags, (void *)b->d_sc, (void *)b->d_b,
                     (void *)b->d_x, (void *)b->d_res, b->d_inner, (void *)b->d_part,
                     (void *)b->d_sysout})
@@ -281,8 +336,10 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     npbcg_batch_default_opts(&o);
     if (opts) o = *opts;
     if ((o.nrhs != 1 && o.nrhs != 2 && o.nrhs != 4) ||
-        (o.mode != NPBCG_SOLVE_FP64 && o.mode != NPBCG_SOLVE_MIXED))
-        return set_error(NPBCG_ERR_INVALID, "npbcg_batch_create: nrhs must be 1, 2 or 4");
+        (o.mode != NPBCG_SOLVE_FP64 && o.mode != NPBCG_SOLVE_MIXED) || o.layout < 0 ||
+        o.layout > 2)
+        return set_error(NPBCG_ERR_INVALID,
+                         "npbcg_batch_create: nrhs must be 1, 2 or 4; mode and layout valid");
     long long rows = 0, nnz = 0;
     for (int s = 0; s < nsys; s++) {
         if (n[s] < 1 || !rowstr[s] || !colidx[s] || !a[s] || rowstr[s][0] != 0)
@@ -328,25 +385,61 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     TRY(hipStreamCreate(&b->stream));
     TRY(hipEventCreate(&b->ev0));
     TRY(hipEventCreate(&b->ev1));
-    // global CSR: rows concatenated, columns offset by the system's first row
+    // global CSR: rows concatenated, columns offset by the system's first
+    // row, each row's entries in column order (the column blocks need it)
     std::vector<int> rp((size_t)rows + 1), col((size_t)nnz);
     std::vector<double> val((size_t)nnz);
     std::vector<float> val32((size_t)nnz);
-    long long e = 0;
+    b->cbw = cb_width(b, b->device);
+    b->sys_nb.resize(nsys);
+    std::vector<long long> sys_seg(nsys), sys_part(nsys);
+    long long seg_total = 0, part_total = 0;
     for (int s = 0; s < nsys; s++) {
-        const int off = b->row_off[s];
+        b->sys_nb[s] = (n[s] + b->cbw - 1) / b->cbw;
+        sys_seg[s] = seg_total;
+        sys_part[s] = part_total;
+        seg_total += (long long)n[s] * (b->sys_nb[s] + 1);
+        part_total += (long long)n[s] * b->sys_nb[s] * b->nr;
+    }
+    std::vector<int> seg((size_t)seg_total);
+    long long e = 0;
+    std::vector<int> order;
+    for (int s = 0; s < nsys; s++) {
+        const int off = b->row_off[s], nb = b->sys_nb[s];
         for (int j = 0; j < n[s]; j++) {
             rp[off + j] = (int)e;
-            for (int k = rowstr[s][j]; k < rowstr[s][j + 1]; k++, e++) {
-                col[e] = colidx[s][k] + off;
+            const int k0 = rowstr[s][j], k1 = rowstr[s][j + 1];
+            order.resize(k1 - k0);
+            for (int k = k0; k < k1; k++) order[k - k0] = k;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int x, int y) { return colidx[s][x] < colidx[s][y]; });
+            int* sg = &seg[(size_t)(sys_seg[s] + (long long)j * (nb + 1))];
+            int blk = 0;
+            for (int k : order) {
+                const int c = colidx[s][k];
+                while (blk < nb && c >= blk * b->cbw) sg[blk++] = (int)e;
+                col[e] = c + off;
                 val[e] = a[s][k];
                 val32[e] = (float)a[s][k];
+                e++;
             }
+            while (blk <= nb) sg[blk++] = (int)e;
         }
     }
     rp[rows] = (int)e;
     const size_t m = (size_t)rows * b->nr;
     b->tiles_cap = max_tiles(b, kMinTile);
+    b->items_cap = 0;
+    for (int s = 0; s < nsys; s++) b->items_cap += b->sys_nb[s] * ((n[s] + kMinTile - 1) / kMinTile);
+    b->part_cap = std::max(b->tiles_cap, b->items_cap);
+    std::vector<int> sysinfo(3 * (size_t)nsys + 2);  // sys_item (filled per shape) | sys_row | sys_nb
+    for (int s = 0; s <= nsys; s++) sysinfo[nsys + 1 + s] = b->row_off[s];
+    for (int s = 0; s < nsys; s++) sysinfo[2 * nsys + 2 + s] = b->sys_nb[s];
+    std::vector<long long> sys64(2 * (size_t)nsys);
+    for (int s = 0; s < nsys; s++) {
+        sys64[s] = sys_seg[s];
+        sys64[nsys + s] = sys_part[s];
+    }
     TRY(hipMalloc(&b->d_rowptr, sizeof(int) * (rows + 1)));
     TRY(hipMalloc(&b->d_col, sizeof(int) * std::max<long long>(nnz, 1)));
     TRY(hipMalloc(&b->d_val, sizeof(double) * std::max<long long>(nnz, 1)));
@@ -360,7 +453,18 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     TRY(hipMalloc(&b->d_res, sizeof(double) * m));
     const size_t inner_elem = o.mode == NPBCG_SOLVE_MIXED ? sizeof(float) : sizeof(double);
     TRY(hipMalloc(&b->d_inner, inner_elem * 4 * m));
-    TRY(hipMalloc(&b->d_part, sizeof(double) * 4 * (size_t)b->tiles_cap * b->nr));
+    TRY(hipMalloc(&b->d_part, sizeof(double) * 4 * (size_t)b->part_cap * b->nr));
+    TRY(hipMalloc(&b->d_items, sizeof(int) * 4 * (size_t)std::max(b->items_cap, 1)));
+    TRY(hipMalloc(&b->d_sysinfo, sizeof(int) * sysinfo.size()));
+    TRY(hipMalloc(&b->d_sys64, sizeof(long long) * sys64.size()));
+    TRY(hipMalloc(&b->d_seg, sizeof(int) * std::max<long long>(seg_total, 1)));
+    TRY(hipMalloc(&b->d_rows_part, inner_elem * std::max<long long>(part_total, 1)));
+    TRY(hipMemcpyAsync(b->d_sysinfo, sysinfo.data(), sizeof(int) * sysinfo.size(),
+                       hipMemcpyHostToDevice, b->stream));
+    TRY(hipMemcpyAsync(b->d_sys64, sys64.data(), sizeof(long long) * sys64.size(),
+                       hipMemcpyHostToDevice, b->stream));
+    TRY(hipMemcpyAsync(b->d_seg, seg.data(), sizeof(int) * seg_total, hipMemcpyHostToDevice,
+                       b->stream));
     TRY(hipMalloc(&b->d_sysout, sizeof(double) * nsys * b->nr));
     TRY(hipMemcpyAsync(b->d_rowptr, rp.data(), sizeof(int) * (rows + 1), hipMemcpyHostToDevice,
                        b->stream));
@@ -383,6 +487,22 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     b->bd.done = b->d_flags;
     b->bd.done_next = b->d_flags + nsys * b->nr;
     b->bd.sc = b->d_sc;
+    b->bd.cbw = b->cbw;
+    b->bd.item = b->d_items;
+    b->bd.sys_item = b->d_sysinfo;
+    b->bd.sys_row = b->d_sysinfo + nsys + 1;
+    b->bd.sys_nb = b->d_sysinfo + 2 * nsys + 2;
+    b->bd.sys_seg = b->d_sys64;
+    b->bd.sys_part = b->d_sys64 + nsys;
+    b->bd.seg = b->d_seg;
+    {
+        const int lds = b->cbw * b->nr * (o.mode == NPBCG_SOLVE_MIXED ? 4 : 8);
+        hipError_t e2 = o.mode == NPBCG_SOLVE_MIXED ? batch_prepare_cb<float>(lds)
+                                                    : batch_prepare_cb<double>(lds);
+        if (e2 != hipSuccess)
+            return fail(set_error(NPBCG_ERR_DEVICE, std::string("batch_prepare_cb: ") +
+                                                        hipGetErrorString(e2)));
+    }
     int lpr = o.lanes_per_row, rt = o.rows_per_tile;
     int dl, dr;
     default_shape(b, &dl, &dr);
@@ -390,7 +510,7 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     if (rt == 0) rt = dr;
     if ((lpr & (lpr - 1)) || lpr < 4 || lpr > 64 || rt < kMinTile || rt > 4096)
         return fail(set_error(NPBCG_ERR_INVALID, "npbcg_batch_create: invalid launch shape"));
-    int rc = apply_shape(b, lpr, rt);
+    int rc = apply_shape(b, lpr, rt, o.layout == 1 ? 1 : 2);
     if (rc != NPBCG_OK) return fail(rc);
     *out = b;
     clear_error();
@@ -413,30 +533,32 @@ int npbcg_batch_tune(npbcg_batch *b, const char *cache_path, int *lanes_per_row,
     if (!b) return set_error(NPBCG_ERR_INVALID, "npbcg_batch_tune: NULL");
     B_HIP(hipSetDevice(b->device));
     const std::string key = shape_key(b);
-    int lpr = 0, rt = 0;
+    int lpr = 0, rt = 0, layout = 2;
     double best = 0.0;
-    if (!cache_lookup(cache_path, key, &lpr, &rt, &best)) {
+    if (!cache_lookup(cache_path, key, &lpr, &rt, &layout, &best)) {
         // All solves marked running so every tile does its work; vectors hold
         // whatever they hold (timing only); state is reset afterwards.
         B_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * 2 * b->nsys * b->nr, b->stream));
         best = 1e30;
-        for (int cl : {4, 8, 16, 32, 64})
-            for (int cr : {64, 128, 256, 512, 1024, 2048}) {
-                int rc = apply_shape(b, cl, cr);
-                if (rc != NPBCG_OK) return rc;
-                double t = 0;
-                rc = b->opts.mode == NPBCG_SOLVE_MIXED ? time_spmm<float>(b, 5, &t)
-                                                       : time_spmm<double>(b, 5, &t);
-                if (rc != NPBCG_OK) return rc;
-                if (t < best) {
-                    best = t;
-                    lpr = cl;
-                    rt = cr;
+        for (int cy : {1, 2})
+            for (int cl : {4, 8, 16, 32, 64})
+                for (int cr : {64, 128, 256, 512, 1024, 2048, 4096}) {
+                    int rc = apply_shape(b, cl, cr, cy);
+                    if (rc != NPBCG_OK) return rc;
+                    double t = 0;
+                    rc = b->opts.mode == NPBCG_SOLVE_MIXED ? time_spmm<float>(b, 5, &t)
+                                                           : time_spmm<double>(b, 5, &t);
+                    if (rc != NPBCG_OK) return rc;
+                    if (t < best) {
+                        best = t;
+                        lpr = cl;
+                        rt = cr;
+                        layout = cy;
+                    }
                 }
-            }
-        cache_store(cache_path, key, lpr, rt, best);
+        cache_store(cache_path, key, lpr, rt, layout, best);
     }
-    int rc = apply_shape(b, lpr, rt);
+    int rc = apply_shape(b, lpr, rt, layout);
     if (rc != NPBCG_OK) return rc;
     if (lanes_per_row) *lanes_per_row = lpr;
     if (rows_per_tile) *rows_per_tile = rt;
@@ -513,6 +635,7 @@ int npbcg_batch_solve(npbcg_batch *b, double rtol, int maxit, npbcg_batch_stats 
     st.sweeps = sweeps;
     st.lanes_per_row = b->bd.lpr;
     st.rows_per_tile = b->opts.rows_per_tile;
+    st.layout = b->opts.layout;
     st.time_ms = ms;
     for (int i = 0; i < ns; i++) {
         const double rel = bb[i] > 0.0 ? std::sqrt(rr[i] / bb[i]) : 0.0;

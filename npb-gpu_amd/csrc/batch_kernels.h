@@ -34,6 +34,21 @@ struct BatchDev {
     const int *sys_tile = nullptr;  // [nsys + 1] first tile of each system
     const int *rowptr = nullptr;    // [rows + 1]
     const int *col = nullptr;       // global column indices
+    // Column-blocked SpMM (cb = 1): system s's columns are cut into blocks of
+    // cbw columns whose vector entries are staged in LDS; one workgroup per
+    // item (system, column block, row range) writes the block's row partials,
+    // which the update kernel adds up (q is never stored).
+    int cb = 0;
+    int cbw = 0;                    // column block width
+    int items = 0;
+    const int *item = nullptr;      // [items][4]: system, block, first row, end row (global)
+    const int *sys_item = nullptr;  // [nsys + 1] first item of each system
+    const int *sys_row = nullptr;   // [nsys + 1] first global row of each system
+    const int *sys_nb = nullptr;    // [nsys] column blocks of each system
+    const long long *sys_seg = nullptr;   // [nsys] start of the system's segment table
+    const long long *sys_part = nullptr;  // [nsys] start of the system's row partials
+    const int *seg = nullptr;       // per row, nb + 1 entry offsets: block b's entries
+                                    // are [seg[b], seg[b+1]) (columns sorted in a row)
     int *done = nullptr;            // [nsys * nr] right-hand side finished
     int *done_next = nullptr;       // flags set during a step, published after it
     BatchScalars *sc = nullptr;     // [nsys * nr]
@@ -44,6 +59,7 @@ template <typename T>
 struct BatchVecs {
     const T *val = nullptr;   // matrix values in type T
     T *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    T *rows_part = nullptr;   // column-blocked SpMM: per-block row partials
     double *part_init = nullptr, *part_pq = nullptr, *part_rr[2] = {nullptr, nullptr};
 };
 
@@ -51,6 +67,9 @@ struct BatchVecs {
 template <typename T>
 hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, int first,
                          hipStream_t st);
+// Allows `bytes` of dynamic LDS for the column-blocked SpMM kernels of type T.
+template <typename T>
+hipError_t batch_prepare_cb(int bytes);
 // q = A p and p.q partials only (the SpMM the autotuner times).
 template <typename T>
 hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st);

@@ -179,11 +179,91 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_kernel(BatchDev bd, const T
     block_sum_store<NR>(dot, pq_part + (size_t)t * NR);
 }
 
+// Column-blocked SpMM: item (system s, column block b, rows [row0, row1)).
+// The block's vector entries (cbw rows of p, NR values each) are staged in
+// LDS, then LPR lanes per row accumulate the row's entries of block b and the
+// row partial is stored; p.q partials follow by linearity,
+// p.q = sum_b sum_rows p_row (A_b p)_row.
+template <typename T, int NR, int LPR>
+__global__ __launch_bounds__(kBlock) void batch_spmm_cb_kernel(BatchDev bd,
+                                                               const T *__restrict__ val,
+                                                               const T *__restrict__ p,
+                                                               T *__restrict__ rows_part,
+                                                               double *__restrict__ pq_part) {
+    extern __shared__ double2 lds_cb[];  // 16-byte aligned
+    T *vlds = reinterpret_cast<T *>(lds_cb);
+    const int it = blockIdx.x;
+    const int s = bd.item[4 * it];
+    if (system_done<NR>(bd.done, s)) return;
+    const int b = bd.item[4 * it + 1], row0 = bd.item[4 * it + 2], row1 = bd.item[4 * it + 3];
+    const int srow = bd.sys_row[s], ns = bd.sys_row[s + 1] - srow;
+    const int nb = bd.sys_nb[s];
+    const int c0 = b * bd.cbw;
+    const int w = min(bd.cbw, ns - c0);
+    {
+        const T *src = p + (size_t)(srow + c0) * NR;
+        for (int i = threadIdx.x; i < w; i += kBlock) {
+            const VecT<T, NR> v = load_vec<T, NR>(src + (size_t)i * NR);
+#pragma unroll
+            for (int k = 0; k < NR; k++) vlds[i * NR + k] = v.v[k];
+        }
+    }
+    __syncthreads();
+    const int grp = threadIdx.x / LPR, sub = threadIdx.x % LPR;
+    constexpr int kGroups = kBlock / LPR;
+    const int *segs = bd.seg + bd.sys_seg[s];
+    T *part = rows_part + bd.sys_part[s] + (size_t)b * ns * NR;
+    const int col0 = srow + c0;  // global column of the block's first entry
+    double dot[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) dot[k] = 0.0;
+    for (int row = row0 + grp; row < row1; row += kGroups) {
+        const int local = row - srow;
+        const int e0 = segs[(size_t)local * (nb + 1) + b], e1 = segs[(size_t)local * (nb + 1) + b + 1];
+        T acc[NR];
+#pragma unroll
+        for (int k = 0; k < NR; k++) acc[k] = T(0);
+        int e = e0 + sub;
+        for (; e + 3 * LPR < e1; e += 4 * LPR) {
+            T a[4];
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                a[u] = val[e + u * LPR];
+                c[u] = bd.col[e + u * LPR] - col0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int k = 0; k < NR; k++) acc[k] = fma(a[u], vlds[c[u] * NR + k], acc[k]);
+        }
+        for (; e < e1; e += LPR) {
+            const T a = val[e];
+            const int c = bd.col[e] - col0;
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] = fma(a, vlds[c * NR + k], acc[k]);
+        }
+#pragma unroll
+        for (int m = LPR / 2; m >= 1; m >>= 1)
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] += __shfl_xor(acc[k], m, kWave);
+        if (sub == 0) {
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                part[(size_t)local * NR + k] = acc[k];
+                dot[k] += (double)p[(size_t)row * NR + k] * (double)acc[k];
+            }
+        }
+    }
+    block_sum_store<NR>(dot, pq_part + (size_t)it * NR);
+}
+
 // alpha = rho / p.q per RHS of the tile's system; d += alpha p, r -= alpha q;
 // partials of the new r.r.  The first step of a solve sets tol2 = rtol2 * rho.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void batch_update_kernel(BatchDev bd, const T *__restrict__ p,
                                                               const T *__restrict__ q,
+                                                              const T *__restrict__ rows_part,
                                                               T *__restrict__ d, T *__restrict__ r,
                                                               const double *__restrict__ rho_part,
                                                               const double *__restrict__ pq_part,
@@ -195,7 +275,10 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(BatchDev bd, const
     const int t0 = bd.sys_tile[s], t1 = bd.sys_tile[s + 1];
     double rho[NR], pq[NR];
     system_sum<NR>(rho_part, t0, t1, rho);
-    system_sum<NR>(pq_part, t0, t1, pq);
+    if (bd.cb)
+        system_sum<NR>(pq_part, bd.sys_item[s], bd.sys_item[s + 1], pq);
+    else
+        system_sum<NR>(pq_part, t0, t1, pq);
     T alpha[NR];
     bool act[NR];
 #pragma unroll
@@ -219,12 +302,28 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(BatchDev bd, const
     double acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    const int srow = bd.cb ? bd.sys_row[s] : 0;
+    const int ns = bd.cb ? bd.sys_row[s + 1] - srow : 0;
+    const int nb = bd.cb ? bd.sys_nb[s] : 0;
+    const T *part = bd.cb ? rows_part + bd.sys_part[s] : nullptr;
     for (int row = row0 + threadIdx.x; row < row1; row += kBlock) {
+        T qv[NR];
+        if (bd.cb) {  // q = sum of the column blocks' row partials, blocks in order
+            const size_t local = (size_t)(row - srow) * NR;
+#pragma unroll
+            for (int k = 0; k < NR; k++) qv[k] = part[local + k];
+            for (int b = 1; b < nb; b++)
+#pragma unroll
+                for (int k = 0; k < NR; k++) qv[k] += part[(size_t)b * ns * NR + local + k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NR; k++) qv[k] = q[(size_t)row * NR + k];
+        }
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const size_t at = (size_t)row * NR + k;
             d[at] = fma(alpha[k], p[at], d[at]);
-            const T rv = fma(-alpha[k], q[at], r[at]);
+            const T rv = fma(-alpha[k], qv[k], r[at]);
             r[at] = rv;
             acc[k] += (double)rv * (double)rv;
         }
@@ -421,16 +520,21 @@ hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, in
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
-                                                                        v.part_pq);
+            if (bd.cb)
+                batch_spmm_cb_kernel<T, NR, LPR>
+                    <<<bd.items, kBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
+                        bd, v.val, v.p, v.rows_part, v.part_pq);
+            else
+                batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
+                                                                            v.part_pq);
             return hipGetLastError();
         });
     });
     if (e != hipSuccess) return e;
     e = with_nr(bd.nr, [&](auto nrc) {
         constexpr int NR = decltype(nrc)::value;
-        batch_update_kernel<T, NR><<<tiles, kBlock, 0, st>>>(bd, v.p, v.q, v.d, v.r, rho,
-                                                             v.part_pq, rr, first);
+        batch_update_kernel<T, NR><<<tiles, kBlock, 0, st>>>(bd, v.p, v.q, v.rows_part, v.d, v.r,
+                                                             rho, v.part_pq, rr, first);
         hipError_t e2 = hipGetLastError();
         if (e2 != hipSuccess) return e2;
         batch_pupdate_kernel<T, NR><<<tiles, kBlock, 0, st>>>(bd, v.r, v.p, rr, rho);
@@ -448,11 +552,37 @@ hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st)
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
-                                                                        v.part_pq);
+            if (bd.cb)
+                batch_spmm_cb_kernel<T, NR, LPR>
+                    <<<bd.items, kBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
+                        bd, v.val, v.p, v.rows_part, v.part_pq);
+            else
+                batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
+                                                                            v.part_pq);
             return hipGetLastError();
         });
     });
+}
+
+// Raises the dynamic LDS limit of the column-blocked SpMM instances.
+template <typename T>
+hipError_t batch_prepare_cb(int bytes) {
+    hipError_t last = hipSuccess;
+    auto set = [&](const void *f) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) last = e;
+    };
+#define NPBCG_CB_SET(NR)                                                          \
+    set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 4>));        \
+    set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 8>));        \
+    set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 16>));       \
+    set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 32>));       \
+    set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 64>));
+    NPBCG_CB_SET(1)
+    NPBCG_CB_SET(2)
+    NPBCG_CB_SET(4)
+#undef NPBCG_CB_SET
+    return last;
 }
 
 template <typename T>
@@ -499,6 +629,8 @@ template hipError_t batch_cg_step<float>(const BatchDev &, const BatchVecs<float
                                          hipStream_t);
 template hipError_t batch_cg_step<double>(const BatchDev &, const BatchVecs<double> &, int, int,
                                           hipStream_t);
+template hipError_t batch_prepare_cb<float>(int);
+template hipError_t batch_prepare_cb<double>(int);
 template hipError_t batch_spmm<float>(const BatchDev &, const BatchVecs<float> &, hipStream_t);
 template hipError_t batch_spmm<double>(const BatchDev &, const BatchVecs<double> &, hipStream_t);
 template hipError_t batch_init<float>(const BatchDev &, const BatchVecs<float> &, const double *,

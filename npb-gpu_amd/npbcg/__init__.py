@@ -60,13 +60,15 @@ NET_FEATURES = 8
 class BatchOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("nrhs", ctypes.c_int), ("mode", ctypes.c_int),
                 ("lanes_per_row", ctypes.c_int), ("rows_per_tile", ctypes.c_int),
-                ("inner_steps", ctypes.c_int), ("reserved", ctypes.c_int * 10)]
+                ("inner_steps", ctypes.c_int), ("layout", ctypes.c_int),
+                ("reserved", ctypes.c_int * 9)]
 
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("solves", ctypes.c_int), ("converged", ctypes.c_int), ("sweeps", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("lanes_per_row", ctypes.c_int),
-                ("rows_per_tile", ctypes.c_int), ("reserved_i", ctypes.c_int * 2),
+                ("rows_per_tile", ctypes.c_int), ("layout", ctypes.c_int),
+                ("reserved_i", ctypes.c_int),
                 ("time_ms", ctypes.c_double), ("max_relres", ctypes.c_double),
                 ("reserved_d", ctypes.c_double * 4)]
 
@@ -519,7 +521,8 @@ class Batch:
     """Independent systems solved together (npbcg_batch_*)."""
 
     def __init__(self, mats, nrhs: int = 1, mode: int = SOLVE_MIXED, device: int = 0,
-                 lanes_per_row: int = 0, rows_per_tile: int = 0, inner_steps: int = 0):
+                 lanes_per_row: int = 0, rows_per_tile: int = 0, inner_steps: int = 0,
+                 layout: int = 0):
         self.mats = list(mats)
         self.nrhs = nrhs
         self.n = np.array([m.n for m in self.mats], np.int32)
@@ -527,6 +530,7 @@ class Batch:
         LIB.npbcg_batch_default_opts(ctypes.byref(o))
         o.device, o.nrhs, o.mode = device, nrhs, mode
         o.lanes_per_row, o.rows_per_tile, o.inner_steps = lanes_per_row, rows_per_tile, inner_steps
+        o.layout = layout
         k = len(self.mats)
         rs = (ctypes.c_void_p * k)(*[m.rowstr.ctypes.data for m in self.mats])
         ci = (ctypes.c_void_p * k)(*[m.colidx.ctypes.data for m in self.mats])

@@ -27,11 +27,12 @@ def _rhs(mats, nrhs):
             for s, m in enumerate(mats)]
 
 
-@pytest.mark.parametrize("mode,nrhs", [(1, 4), (0, 4), (1, 1), (0, 2)])
-def test_batch_solves_match_oracle(gpu, mode, nrhs):
+@pytest.mark.parametrize("mode,nrhs,layout", [(1, 4, 2), (0, 4, 2), (1, 1, 2), (0, 2, 2),
+                                              (1, 4, 1), (0, 1, 1)])
+def test_batch_solves_match_oracle(gpu, mode, nrhs, layout):
     mats = _systems(gpu)
     rhs = _rhs(mats, nrhs)
-    with gpu.Batch(mats, nrhs=nrhs, mode=mode) as bt:
+    with gpu.Batch(mats, nrhs=nrhs, mode=mode, layout=layout) as bt:
         bt.set_rhs(rhs)
         st, steps, rel = bt.solve(RTOL, 3000)
         xs = bt.solution()
@@ -54,10 +55,13 @@ def test_batch_launch_shapes_agree(gpu):
     mats = _systems(gpu, 3, seed=9)
     rhs = _rhs(mats, 4)
     sols = []
-    for lpr, rt in [(4, 64), (16, 256), (64, 1024)]:
-        with gpu.Batch(mats, nrhs=4, mode=0, lanes_per_row=lpr, rows_per_tile=rt) as bt:
+    for lpr, rt, layout in [(4, 64, 1), (16, 256, 1), (64, 1024, 1), (8, 64, 2), (16, 512, 2),
+                            (64, 4096, 2)]:
+        with gpu.Batch(mats, nrhs=4, mode=0, lanes_per_row=lpr, rows_per_tile=rt,
+                       layout=layout) as bt:
             bt.set_rhs(rhs)
-            bt.solve(RTOL, 3000)
+            st, _, _ = bt.solve(RTOL, 3000)
+            assert st.layout == layout and st.converged == st.solves
             sols.append(np.concatenate(bt.solution()))
     for x in sols[1:]:
         assert np.max(np.abs(x - sols[0])) <= 1e-9 * np.max(np.abs(sols[0]))
