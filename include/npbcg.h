@@ -366,6 +366,9 @@ int npbcg_batch_get_solution(npbcg_batch *batch, double *x);
  * receives the chosen SpMM time. */
 int npbcg_batch_tune(npbcg_batch *batch, const char *cache_path, int *lanes_per_row,
                      int *rows_per_tile, double *ms);
+/* Mean device time of the batch's SpMM (current launch shape) over reps
+ * launches, every system marked running. */
+int npbcg_batch_spmm_bench(npbcg_batch *batch, int reps, double *avg_ms);
 /* Generator threads of npbcg_makea calls made from the calling thread
  * (0: the default, NPBCG_GEN_THREADS or up to 16). */
 void npbcg_set_generator_threads(int threads);

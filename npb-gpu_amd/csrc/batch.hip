@@ -653,6 +653,17 @@ int npbcg_batch_solve(npbcg_batch *b, double rtol, int maxit, npbcg_batch_stats 
     return NPBCG_OK;
 }
 
+int npbcg_batch_spmm_bench(npbcg_batch *b, int reps, double *avg_ms) {
+    if (!b || reps < 1 || !avg_ms) return set_error(NPBCG_ERR_INVALID, "npbcg_batch_spmm_bench: invalid argument");
+    B_HIP(hipSetDevice(b->device));
+    B_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * 2 * b->nsys * b->nr, b->stream));
+    int rc = b->opts.mode == NPBCG_SOLVE_MIXED ? time_spmm<float>(b, reps, avg_ms)
+                                               : time_spmm<double>(b, reps, avg_ms);
+    if (rc != NPBCG_OK) return rc;
+    clear_error();
+    return NPBCG_OK;
+}
+
 int npbcg_batch_get_solution(npbcg_batch *b, double *x) {
     if (!b || !x) return set_error(NPBCG_ERR_INVALID, "npbcg_batch_get_solution: NULL");
     B_HIP(hipSetDevice(b->device));

@@ -180,6 +180,7 @@ def _load() -> ctypes.CDLL:
         "npbcg_batch_solve": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int,
                                              ctypes.POINTER(BatchStats), vp, vp]),
         "npbcg_batch_get_solution": (ctypes.c_int, [vp, _F64P]),
+        "npbcg_batch_spmm_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
         "npbcg_batch_tune": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int), _DP]),
         "npbcg_compat_set_n": (ctypes.c_int, [ctypes.c_int]),
@@ -584,6 +585,11 @@ class Batch:
         if check:
             _check(rc, "npbcg_batch_solve")
         return st, steps.reshape(-1, self.nrhs), rel.reshape(-1, self.nrhs)
+
+    def spmm_bench(self, reps: int = 20) -> float:
+        ms = ctypes.c_double(0.0)
+        _check(LIB.npbcg_batch_spmm_bench(self._h, reps, ctypes.byref(ms)), "npbcg_batch_spmm_bench")
+        return ms.value
 
     def solution(self):
         x = np.zeros(self.rows * self.nrhs)
