@@ -327,7 +327,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-outer-iters", type=int, default=20,
+    ap.add_argument("--cpu-outer-iters", type=int, default=75,
                     help="outer iterations in the CPU baseline sample (0 = skip)")
     ap.add_argument("--spmv-variant", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")

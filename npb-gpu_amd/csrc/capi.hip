@@ -788,6 +788,8 @@ int npbcg_describe(const npbcg_ctx *c, char *buf, int len) {
     hipDeviceProp_t prop{};
     if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) std::strcpy(prop.name, "unknown");
     prop.name[sizeof(prop.name) - 1] = '\0';
+    prop.gcnArchName[sizeof(prop.gcnArchName) - 1] = '\0';
+    if (prop.name[0] == '\0') std::strncpy(prop.name, prop.gcnArchName, sizeof(prop.name) - 1);
     for (char *q = prop.name; *q; q++)
         if (*q == '"' || *q == '\\') *q = ' ';
     char tmp[768];

@@ -164,21 +164,37 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
             vblk[wb - 1] = nv;
             if (store) p_new[c0 + wb - 1] = nv;
         }
-    } else {
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    // Wave-uniform indices (readfirstlane): every branch below that depends on
+    // them is a scalar branch, so the wavefront stays converged around the
+    // cross-lane reduction.
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int s0 = wg_slice[wg];
+    const int nsl = wg_slice[wg + 1] - s0;
+    // The first UNR matrix entries of this wave's first slice are loaded
+    // before the vector block is staged, so their HBM latency overlaps the
+    // LDS fill (they do not depend on it).
+    double pre_a[UNR];
+    int pre_c[UNR];
+    const bool pre = wave < nsl &&
+                     (slice_off[s0 + wave + 1] - slice_off[s0 + wave]) / kWave >= UNR;
+    if (pre) {
+        const int base = slice_off[s0 + wave];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            pre_a[u] = val[base + lane + (size_t)u * kWave];
+            pre_c[u] = col[base + lane + (size_t)u * kWave];
+        }
+    }
+    if constexpr (!PUPD) {
         const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
 #pragma unroll 4
         for (int i = threadIdx.x; i < wb / 2; i += BLOCK) lds_v2[i] = v2[i];
         if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
     }
     __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    // Wave-uniform indices (readfirstlane): every branch below that depends on
-    // them is a scalar branch, so the wavefront stays converged around the
-    // cross-lane reduction.
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     double *part_b = rows_part + (size_t)b * n;
-    const int s0 = wg_slice[wg];
-    const int nsl = wg_slice[wg + 1] - s0;
     const int rounds = (nsl + kWaves - 1) / kWaves;
     for (int rd = 0; rd < rounds; rd++) {
         // Slices are sorted widest first; alternating the direction of each
@@ -192,6 +208,11 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
         const uint16_t *cp = col + base + lane;
         double acc = 0.0;
         int j = 0;
+        if (rd == 0 && pre) {
+#pragma unroll
+            for (int u = 0; u < UNR; u++) acc = fma(pre_a[u], vblk[pre_c[u]], acc);
+            j = UNR;
+        }
         for (; j + UNR <= width; j += UNR) {
             double av[UNR];
             int c[UNR];

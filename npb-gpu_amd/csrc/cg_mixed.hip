@@ -43,6 +43,24 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     const int b = wg / nrange;
     const int c0 = b * W;
     const int wb = min(W, n - c0);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int s0 = wg_slice[wg];
+    const int nsl = wg_slice[wg + 1] - s0;
+    // First UNR entries of this wave's first slice, loaded before the vector
+    // block is staged (their latency overlaps the LDS fill).
+    float pre_a[UNR];
+    int pre_c[UNR];
+    const bool pre = wave < nsl &&
+                     (slice_off[s0 + wave + 1] - slice_off[s0 + wave]) / kWave >= UNR;
+    if (pre) {
+        const int base = slice_off[s0 + wave];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            pre_a[u] = val[base + lane + (size_t)u * kWave];
+            pre_c[u] = col[base + lane + (size_t)u * kWave];
+        }
+    }
     {
         // c0 is even (W even); stage pairs, then a possible odd tail.
         const float2 *v2 = reinterpret_cast<const float2 *>(v + c0);
@@ -52,11 +70,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
         if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
     }
     __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     float *part_b = rows_part + (size_t)b * n;
-    const int s0 = wg_slice[wg];
-    const int nsl = wg_slice[wg + 1] - s0;
     const int rounds = (nsl + kWaves - 1) / kWaves;
     for (int rd = 0; rd < rounds; rd++) {
         const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
@@ -68,6 +82,11 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
         const uint16_t *cp = col + base + lane;
         float acc = 0.0f;
         int j = 0;
+        if (rd == 0 && pre) {
+#pragma unroll
+            for (int u = 0; u < UNR; u++) acc = fmaf(pre_a[u], vblk[pre_c[u]], acc);
+            j = UNR;
+        }
         for (; j + UNR <= width; j += UNR) {
             float av[UNR];
             int c[UNR];
