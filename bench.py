@@ -159,7 +159,9 @@ def pmc_summary(fetch_csv, write_csv, out_path):
     write = _pmc_rows(write_csv, "WRITE_SIZE")
 
     def pick(d, key):
-        ks = [k for k in d if key in k]
+        # the kernel of that name with the most dispatches (the CG SpMV, not
+        # the once-per-iteration residual SpMV of the same template)
+        ks = sorted((k for k in d if key in k), key=lambda k: -len(d[k]))
         return (ks[0], d[ks[0]]) if ks else (None, [])
 
     _, cf = pick(fetch, "copy_kernel")
@@ -179,7 +181,18 @@ def pmc_summary(fetch_csv, write_csv, out_path):
            "copy_bytes_each_way": PROFILE_COPY_BYTES,
            "fetch_scale": f_scale, "write_scale": w_scale,
            "spmv_fetch_bytes": spmv_f, "spmv_write_bytes": spmv_w,
-           "bytes_per_launch": spmv_f + spmv_w}
+           "bytes_per_launch": spmv_f + spmv_w,
+           "workload": "n=75000 nonzer=13 shift=60 seed=271828, FP64, SELL column-blocked SpMV q = A p",
+           "commands": [
+               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof_fetch -o fetch "
+               "-- python3 bench.py --profile-pass",
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof_write -o write "
+               "-- python3 bench.py --profile-pass",
+               "python3 bench.py --pmc-summary <fetch csv> <write csv> " + out_path],
+           "note": "FETCH_SIZE/WRITE_SIZE are L2-to-fabric KiB per dispatch; each is divided by "
+                   "what it reads for the copy kernel of known traffic (fetch_scale, write_scale). "
+                   "The matrix fits the 256 MB Infinity Cache, so fetched bytes include hits "
+                   "there, not only HBM reads."}
     with open(out_path, "w") as f:
         json.dump(doc, f, indent=1)
     print(json.dumps(doc))
