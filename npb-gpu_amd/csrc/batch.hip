@@ -383,8 +383,8 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     } while (0)
     TRY(hipSetDevice(b->device));
     TRY(hipStreamCreate(&b->stream));
-    TRY(hipEventCreate(&b->ev0));
-    TRY(hipEventCreate(&b->ev1));
+    TRY(hipEventCreateWithFlags(&b->ev0, hipEventDisableSystemFence));
+    TRY(hipEventCreateWithFlags(&b->ev1, hipEventDisableSystemFence));
     // global CSR: rows concatenated, columns offset by the system's first
     // row, each row's entries in column order (the column blocks need it)
     std::vector<int> rp((size_t)rows + 1), col((size_t)nnz);

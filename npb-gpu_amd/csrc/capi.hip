@@ -313,8 +313,8 @@ int ensure_probes(npbcg_ctx *c) {
     const size_t want = (size_t)c->cgitmax;
     while (c->probe_start.size() < want) {
         hipEvent_t a, b;
-        NPBCG_HIP(hipEventCreate(&a));
-        NPBCG_HIP(hipEventCreate(&b));
+        NPBCG_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+        NPBCG_HIP(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
         c->probe_start.push_back(a);
         c->probe_stop.push_back(b);
     }
@@ -411,8 +411,8 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
 
     CREATE_HIP(hipSetDevice(c->device));
     CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    CREATE_HIP(hipEventCreate(&c->ev_start));
-    CREATE_HIP(hipEventCreate(&c->ev_stop));
+    CREATE_HIP(hipEventCreateWithFlags(&c->ev_start, hipEventDisableSystemFence));
+    CREATE_HIP(hipEventCreateWithFlags(&c->ev_stop, hipEventDisableSystemFence));
 
     hipDeviceProp_t prop;
     CREATE_HIP(hipGetDeviceProperties(&prop, c->device));

@@ -277,8 +277,8 @@ int npbcg_net_create(npbcg_net **out, int device, unsigned long long seed) {
     NET_TRY(hipMalloc(&net->norm, sizeof(float) * 2 * kFeat));
     NET_TRY(hipMalloc(&net->d_loss, sizeof(double) * 2));
     NET_TRY(hipHostMalloc(&net->h_loss, sizeof(double) * 2, hipHostMallocDefault));
-    NET_TRY(hipEventCreate(&net->ev0));
-    NET_TRY(hipEventCreate(&net->ev1));
+    NET_TRY(hipEventCreateWithFlags(&net->ev0, hipEventDisableSystemFence));
+    NET_TRY(hipEventCreateWithFlags(&net->ev1, hipEventDisableSystemFence));
     net->nb.norm = net->norm;
     std::vector<float> p;
     init_params(p, seed);
