@@ -130,6 +130,10 @@ struct npbcg_ctx {
         int loaded = -1;         // preconditioner kind whose values op holds
     } g;
 
+    // step budgets learned from the last converged solve of each mode (0: the
+    // defaults): FP64 steps of the first batch; mixed inner steps of sweeps 1, 2
+    int budget[2][2] = {{0, 0}, {0, 0}};
+
     // captured solve sequences, one per mode; replayed while the key matches
     struct SolveGraph {
         hipGraph_t g = nullptr;
@@ -865,11 +869,11 @@ int ensure_mixed(npbcg_ctx *c) {
     const size_t stored = (size_t)std::max<long long>(1, c->sell_stored);
     const size_t vb = (size_t)c->pb.vec_blocks;
     NPBCG_HIP(hipMalloc(&c->d_val32, sizeof(float) * stored));
-    NPBCG_HIP(hipMalloc(&c->d_vec32, sizeof(float) * 4 * c->vstride));
+    NPBCG_HIP(hipMalloc(&c->d_vec32, sizeof(float) * 5 * c->vstride));
     NPBCG_HIP(hipMalloc(&c->d_rows32, sizeof(float) * (size_t)sd.nblk * c->n));
     NPBCG_HIP(hipMalloc(&c->d_part32, sizeof(double) * (5 * vb + (size_t)sd.grid)));
     NPBCG_HIP(hipMalloc(&c->d_sc32, npbcg_dev::scalars_bytes()));
-    NPBCG_HIP(hipMemsetAsync(c->d_vec32, 0, sizeof(float) * 4 * c->vstride, c->stream));
+    NPBCG_HIP(hipMemsetAsync(c->d_vec32, 0, sizeof(float) * 5 * c->vstride, c->stream));
     NPBCG_HIP(hipMemsetAsync(c->d_sc32, 0, npbcg_dev::scalars_bytes(), c->stream));
     NPBCG_HIP(npbcg_dev::launch_to_fp32(c->d_sell_val, c->d_val32, c->sell_stored, c->stream));
     // FP32 vector block (padded to 4 entries) + one FP64 slot per slice
@@ -882,6 +886,7 @@ int ensure_mixed(npbcg_ctx *c) {
     mp.r = c->d_vec32 + c->vstride;
     mp.p = c->d_vec32 + 2 * c->vstride;
     mp.q = c->d_vec32 + 3 * c->vstride;
+    mp.p_alt = c->d_vec32 + 4 * c->vstride;
     mp.rows_part = c->d_rows32;
     mp.part_xx = c->d_part32;
     mp.part_rr[0] = c->d_part32 + vb;
@@ -1404,7 +1409,10 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
     Scalars sc{};
     double relres = 1.0;
     if (mode == NPBCG_SOLVE_FP64) {
-        int batch = std::min(batch0, maxit);
+        // a repeated solve on this matrix launches one step more than the last
+        // one needed instead of a full batch (steps past the stop return early
+        // but still cost a launch each)
+        int batch = std::min(c->budget[0][0] > 0 ? c->budget[0][0] : batch0, maxit);
         // first batch: reset, init, `batch` steps, residual (captured once per key)
         auto first_batch = [&]() -> hipError_t {
             hipError_t e = npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 1);
@@ -1436,6 +1444,7 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
             first = false;
         }
         s.iterations = sc.steps;
+        if (relres <= rtol && s.sweeps == 1) c->budget[0][0] = std::min(batch0, sc.steps + 1);
     } else {
         const npbcg_dev::MixedProblem &mp = c->mp;
         Fp64Problem inner = pbs;  // same launch shapes, inner scalars
@@ -1444,8 +1453,10 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
         // enqueued before the first host synchronisation (FP32 holds about
         // 1e-6 per sweep, so two reach 1e-10 from x = 0 on well-conditioned
         // systems); later ones go one at a time.  maxit is checked per sweep.
+        const int b1 = c->budget[1][0] > 0 ? c->budget[1][0] : batch0 - 2;
+        const int b2 = c->budget[1][1] > 0 ? c->budget[1][1] : batch0 / 2 + 1;
         auto sweep = [&](int index) -> hipError_t {
-            const int steps = index == 0 ? batch0 - 2 : batch0 / 2 + 1;
+            const int steps = index == 0 ? b1 : index == 1 ? b2 : batch0 / 2 + 1;
             hipError_t e = npbcg_dev::launch_sweep_prepare(pbs.sc, mp.sc, c->rhs_norm, rtol,
                                                            maxit, index == 0 ? 1 : 0, c->stream);
             if (e == hipSuccess)
@@ -1464,9 +1475,9 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
             return e;
         };
         NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
-        int rc0 = run_solve_sequence(c, 1, rtol, maxit, batch0, first_two);
+        int rc0 = run_solve_sequence(c, 1, rtol, maxit, b1 * 1000 + b2, first_two);
         if (rc0 != NPBCG_OK) return rc0;
-        int enqueued = 2, inner_steps = 0;
+        int enqueued = 2, inner_steps = 0, si_last_steps = 0;
         for (bool first = true;; first = false) {
             if (!first) NPBCG_HIP(sweep(enqueued++));
             NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
@@ -1476,11 +1487,16 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
             rc = read_scalars_from(c, pbs.sc, &sc);
             if (rc != NPBCG_OK) return rc;
             inner_steps = sc.inner_total + si.steps;
+            si_last_steps = si.steps;
             relres = sc.rnorm / c->rhs_norm;
             if (relres <= rtol || inner_steps >= maxit || enqueued >= 1000) break;
         }
         s.sweeps = sc.sweeps;
         s.iterations = inner_steps;
+        if (relres <= rtol && s.sweeps == 2) {  // the common case: learn both budgets
+            c->budget[1][0] = std::min(batch0 - 2, std::max(2, sc.sweep1_steps + 1));
+            c->budget[1][1] = std::min(batch0 / 2 + 1, std::max(2, si_last_steps + 1));
+        }
         s.fp32_spmvs = inner_steps;
         s.fp64_spmvs = s.sweeps;
     }

@@ -69,6 +69,8 @@ struct Scalars {
     int skip_resid;  // mixed solve: the residual pass of a sweep that is not needed
     int sweeps;      // mixed solve: refinement sweeps run
     int inner_total; // mixed solve: inner steps of the finished sweeps
+    int sweep1_steps;  // mixed solve: inner steps of the first two sweeps
+    int sweep2_steps;
     int pad;
 };
 

@@ -136,6 +136,7 @@ struct MixedProblem {
     const PrecondOp *pop = nullptr;  // sparse preconditioner (FP32 values); z in pz
     float *pz = nullptr;
     float *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;  // n+2 each
+    float *p_alt = nullptr;       // second p buffer of the fused p update
     float *rows_part = nullptr;   // nblk * n per-block row sums
     double *part_xx = nullptr;    // r0.r0 partials of an inner start [vec_blocks]
     double *part_rr[2] = {nullptr, nullptr};  // r.r partials [vec_blocks]

@@ -27,14 +27,33 @@ __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double *__restric
 
 // q = A p in FP32 over the SELL layout (see spmv_sell_kernel): per-block row
 // sums into rows32[b][row], FP64 partials of p.q.
-template <int BLOCK, int UNR>
+// PUPD: the previous step's p update is done here.  beta = (r.r) / (old r.r)
+// from the partials and the stop test on r.r (one workgroup records it; all
+// workgroups reach the same decision and return); else p = r + beta p_old is
+// formed while the vector block is staged, and the workgroups of row range 0
+// store their block of it in p_new (p_old and p_new are different buffers).
+template <int BLOCK, int UNR, bool PUPD>
 __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     const int *__restrict__ wg_slice, const int *__restrict__ slice_off,
     const int *__restrict__ slice_row, const float *__restrict__ val,
     const uint16_t *__restrict__ col, const float *__restrict__ v,
     float *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W, int nrange,
-    const Scalars *__restrict__ sc) {
+    Scalars *sc, const float *__restrict__ r, float *__restrict__ p_new,
+    const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec) {
     if (sc->done) return;
+    float b32 = 0.0f;
+    if constexpr (PUPD) {
+        const double rho = reduce_partials<BLOCK>(rr_part, n_vec);
+        const double rho0 = reduce_partials<BLOCK>(rho_part, n_vec);
+        const bool stop = fabs(rho) <= sc->tol2;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->rho = rho;
+            sc->beta = rho / rho0;
+            if (stop) sc->done = 1;
+        }
+        if (stop) return;
+        b32 = (float)(rho / rho0);
+    }
     extern __shared__ double2 lds_v2[];  // 16-byte aligned
     float *vblk = reinterpret_cast<float *>(lds_v2);
     double *slice_dot = reinterpret_cast<double *>(vblk + ((W + 3) & ~3));
@@ -61,7 +80,27 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
             pre_c[u] = col[base + lane + (size_t)u * kWave];
         }
     }
-    {
+    if constexpr (PUPD) {
+        const float2 *v2 = reinterpret_cast<const float2 *>(v + c0);
+        const float2 *r2 = reinterpret_cast<const float2 *>(r + c0);
+        float2 *o2 = reinterpret_cast<float2 *>(p_new + c0);
+        float2 *l2 = reinterpret_cast<float2 *>(vblk);
+        const bool store = (wg % nrange) == 0;
+#pragma unroll 4
+        for (int i = threadIdx.x; i < wb / 2; i += BLOCK) {
+            const float2 pv = v2[i], rv = r2[i];
+            float2 nv;
+            nv.x = fmaf(b32, pv.x, rv.x);
+            nv.y = fmaf(b32, pv.y, rv.y);
+            l2[i] = nv;
+            if (store) o2[i] = nv;
+        }
+        if ((wb & 1) && threadIdx.x == 0) {
+            const float nv = fmaf(b32, v[c0 + wb - 1], r[c0 + wb - 1]);
+            vblk[wb - 1] = nv;
+            if (store) p_new[c0 + wb - 1] = nv;
+        }
+    } else {
         // c0 is even (W even); stage pairs, then a possible odd tail.
         const float2 *v2 = reinterpret_cast<const float2 *>(v + c0);
         float2 *l2 = reinterpret_cast<float2 *>(vblk);
@@ -101,7 +140,9 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
         for (; j < width; j++) acc = fmaf(vp[(size_t)j * kWave], vblk[cp[(size_t)j * kWave]], acc);
         const int row = slice_row[(size_t)s * kWave + lane];
         if (row >= 0) part_b[row] = acc;
-        const double d = segment_sum<64>(row >= 0 ? (double)v[row] * (double)acc : 0.0);
+        float pr = 0.0f;
+        if (row >= 0) pr = PUPD ? fmaf(b32, v[row], r[row]) : v[row];
+        const double d = segment_sum<64>((double)pr * (double)acc);
         if (lane == 0) slice_dot[k] = d;
     }
     __syncthreads();
@@ -201,8 +242,12 @@ __global__ void sweep_prepare_kernel(Scalars *outer, Scalars *inner, double rhs_
         if (first) {
             outer->inner_total = 0;
             outer->sweeps = 0;
+            outer->sweep1_steps = 0;
+            outer->sweep2_steps = 0;
         } else {
             outer->inner_total += inner->steps;
+            if (outer->sweeps == 1) outer->sweep1_steps = inner->steps;
+            if (outer->sweeps == 2) outer->sweep2_steps = inner->steps;
         }
         const double rel = first ? 1.0 : outer->rnorm / rhs_norm;
         const bool stop = (!first && rel <= rtol) || outer->inner_total >= maxit;
@@ -263,13 +308,31 @@ __global__ __launch_bounds__(kBlock) void jacobi_kernel(const int *__restrict__ 
 // ---------------------------------------------------------------------------
 
 hipError_t prepare_mixed_kernels(int lds_bytes) {
-    hipError_t e1 = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, 8>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-    hipError_t e2 = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&spmv_sell32_kernel<512, 8>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-    return e1 != hipSuccess ? e1 : e2;
+    hipError_t last = hipSuccess;
+    for (const void *f : {reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, 8, false>),
+                          reinterpret_cast<const void *>(&spmv_sell32_kernel<512, 8, false>),
+                          reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, 8, true>),
+                          reinterpret_cast<const void *>(&spmv_sell32_kernel<512, 8, true>)}) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+        if (e != hipSuccess) last = e;
+    }
+    return last;
+}
+
+template <bool PUPD>
+static hipError_t launch_spmv32(const Fp64Problem &pb, const MixedProblem &mp, const float *v,
+                                float *p_new, const double *rr_part, const double *rho_part,
+                                hipStream_t st) {
+    const SellDev &sd = pb.sell;
+    if (sd.block == 512)
+        spmv_sell32_kernel<512, 8, PUPD><<<sd.grid, 512, mp.lds_bytes, st>>>(
+            sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
+            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks);
+    else
+        spmv_sell32_kernel<1024, 8, PUPD><<<sd.grid, 1024, mp.lds_bytes, st>>>(
+            sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
+            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks);
+    return hipGetLastError();
 }
 
 hipError_t launch_to_fp32(const double *src, float *dst, long long count, hipStream_t st) {
@@ -289,27 +352,28 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && mp.pop)  // p = z = M^-1 r; r0.z0 into rz[1], r0.r0 stays in xx
         e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, mp.p, mp.part_rz[1], vb, nullptr, st);
+    const SellDev &sd = pb.sell;
+    // Plain CG: the p update of step it-1 runs inside step it's SpMV (two
+    // launches per step), p alternating between mp.p and mp.p_alt.
+    const bool fused = !mp.pop && !mp.minv && mp.p_alt;
+    float *pbuf[2] = {mp.p, mp.p_alt};
     for (int it = 0; it < steps && e == hipSuccess; it++) {
-        const SellDev &sd = pb.sell;
-        if (sd.block == 512)
-            spmv_sell32_kernel<512, 8><<<sd.grid, 512, mp.lds_bytes, st>>>(
-                sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, mp.p, mp.rows_part,
-                mp.part_pq, pb.n, sd.W, sd.nrange, mp.sc);
+        float *p_it = fused ? pbuf[it & 1] : mp.p;
+        if (fused && it > 0)
+            e = launch_spmv32<true>(pb, mp, pbuf[(it - 1) & 1], p_it, mp.part_rr[(it - 1) & 1],
+                                    rho32_source(mp, it - 1), st);
         else
-            spmv_sell32_kernel<1024, 8><<<sd.grid, 1024, mp.lds_bytes, st>>>(
-                sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, mp.p, mp.rows_part,
-                mp.part_pq, pb.n, sd.W, sd.nrange, mp.sc);
-        e = hipGetLastError();
+            e = launch_spmv32<false>(pb, mp, p_it, nullptr, nullptr, nullptr, st);
         if (e != hipSuccess) break;
         // sparse preconditioner: alpha from the previous step's r.z (the
         // start's in rz[1]); beta from r.z; the stop test on r.r
         const double *rho_prev = mp.pop ? mp.part_rz[(it + 1) & 1] : rho32_source(mp, it);
-        update_zr32_kernel<<<vb, kBlock, 0, st>>>(mp.p, mp.q, mp.d, mp.r, pb.n, rho_prev,
+        update_zr32_kernel<<<vb, kBlock, 0, st>>>(p_it, mp.q, mp.d, mp.r, pb.n, rho_prev,
                                                   mp.part_pq, sd.grid, mp.part_rr[it & 1],
                                                   mp.rows_part, sd.nblk, mp.sc, it == 0 ? 1 : 0,
                                                   mp.minv, mp.pop ? mp.part_xx : nullptr);
         e = hipGetLastError();
-        if (e != hipSuccess) break;
+        if (e != hipSuccess || fused) continue;
         if (mp.pop) {
             e = launch_precond_apply32(*mp.pop, mp.r, mp.pz, nullptr, mp.part_rz[it & 1], vb,
                                        mp.sc, st);
