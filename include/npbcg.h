@@ -323,11 +323,15 @@ typedef struct npbcg_batch_opts {
     int device;
     int nrhs;            /* 1, 2 or 4 */
     int mode;            /* NPBCG_SOLVE_FP64 or NPBCG_SOLVE_MIXED */
-    int lanes_per_row;   /* SpMM lanes per row: 4, 8, 16, 32, 64; 0 = tuned or default */
-    int rows_per_tile;   /* rows per workgroup tile: 64 .. 4096; 0 = tuned or default */
+    int lanes_per_row;   /* lanes per row of the row-gather kernels: 4, 8, 16, 32, 64;
+                            0 = tuned or default */
+    int rows_per_tile;   /* rows per SpMM workgroup: 64 .. 4096 (layout 3: rounded
+                            down to a multiple of 2048); 0 = tuned or default */
     int inner_steps;     /* MIXED: inner CG steps per sweep at most (0: 60) */
-    int layout;          /* SpMM: 0 = tuned or column blocks, 1 = row gathers,
-                            2 = column blocks (vector blocks staged in LDS) */
+    int layout;          /* SpMM: 0 = tuned or the default (3), 1 = row gathers,
+                            2 = column blocks (vector blocks staged in LDS, rows
+                            in CSR order), 3 = SELL slices over LDS-staged column
+                            blocks (one lane per row, 16-bit columns) */
     int reserved[9];
 } npbcg_batch_opts;
 
@@ -338,7 +342,7 @@ typedef struct npbcg_batch_stats {
     int max_steps;       /* largest inner CG step count of a solve */
     int lanes_per_row;
     int rows_per_tile;
-    int layout;          /* 1 row gathers, 2 column blocks */
+    int layout;          /* 1 row gathers, 2 column blocks, 3 SELL slices */
     int reserved_i;
     double time_ms;      /* HIP events around the device work */
     double max_relres;

@@ -1,6 +1,6 @@
-// Host side of batched solves: block-diagonal upload, tiling, the FP64
-// refinement loop around the batched inner CG, and the SpMM autotuner with
-// its persisted record file.
+// Host side of batched solves: block-diagonal upload, tiling, the SELL copy,
+// the FP64 refinement loop around the batched inner CG, and the SpMM
+// autotuner with its persisted record file.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,9 +13,12 @@
 #include "batch_kernels.h"
 #include "npbcg.h"
 #include "npbcg_internal.h"
+#include "tiled_layout.h"
 
 using namespace npbcg_dev;
+using npbcg_detail::build_sell_blocks;
 using npbcg_detail::clear_error;
+using npbcg_detail::SellBlocks;
 using npbcg_detail::set_error;
 
 #define B_HIP(call)                                                              \
@@ -25,6 +28,17 @@ using npbcg_detail::set_error;
             return set_error(NPBCG_ERR_DEVICE, std::string(#call) + ": " +       \
                                                    hipGetErrorString(_e));       \
     } while (0)
+
+// Host metadata of one SELL copy of the batch.
+struct SellHost {
+    int W = 0;                          // column block width
+    std::vector<int> nb;                // blocks per system
+    std::vector<std::vector<int>> wg;   // per system: global first slice of each
+                                        // (block, window), block-major, + end
+    std::vector<int> nrange;            // windows per system
+    std::vector<long long> part;        // per system: first row partial (elements)
+    long long part_total = 0;
+};
 
 struct npbcg_batch {
     int device = 0;
@@ -53,6 +67,28 @@ struct npbcg_batch {
     long long *d_sys64 = nullptr;    // sys_seg | sys_part
     int *d_seg = nullptr;            // per-row block offsets
     void *d_rows_part = nullptr;     // per-block row partials (inner type)
+    // SELL copy (layout 3), inner type values
+    bool sell_ok = false;
+    SellHost sell_h;
+    void *d_sell_val = nullptr;
+    uint16_t *d_sell_col = nullptr;
+    int *d_sell_off = nullptr, *d_sell_row = nullptr;
+    int *d_block_item = nullptr;     // workgroup -> item of the SELL SpMM
+    size_t block_item_cap = 0;
+    // FP64 SELL copy of the true residual: in FP64 mode the inner copy itself
+    // (aliased), in MIXED mode a copy of its own with blocks sized for FP64
+    // vectors.  Items are one sorting window each.
+    bool res_ok = false, res_owns = false;
+    SellHost res_h;
+    double *d_res_val = nullptr;
+    uint16_t *d_res_col = nullptr;
+    int *d_res_off = nullptr, *d_res_row = nullptr;
+    int *d_res_items = nullptr;      // [items][4]
+    int *d_res_sys = nullptr;        // sys_item | sys_row | sys_nb
+    long long *d_res_part = nullptr; // sys_part
+    int *d_res_block_item = nullptr;
+    int res_items = 0, res_item_blocks = 0;
+    double *d_res_rows_part = nullptr;
     int *d_flags = nullptr;          // done | done_next
     BatchScalars *d_sc = nullptr;
     double *d_b = nullptr, *d_x = nullptr, *d_res = nullptr;   // [rows][nr]
@@ -91,11 +127,12 @@ int max_tiles(const npbcg_batch *b, int rt) {
 constexpr int kMinTile = 64;
 constexpr int kXcds = 8;  // MI355X: workgroup b of a launch runs on XCD b % 8
 
-// Workgroup -> tile order of the gathering kernels.  With at least kXcds
+// Workgroup -> work unit (tile or item) order of the gathering kernels;
+// system s owns units [sys_first[s], sys_first[s+1]).  With at least kXcds
 // systems, whole systems are placed on XCDs (greedy by nonzeros, largest
-// first) so that a system's vector is gathered from one XCD's L2; the lists
-// are interleaved to match the round-robin dispatch and padded with -1.
-// Fewer systems keep the plain order.
+// first) so that a system's vector is read from one XCD's L2; the lists are
+// interleaved to match the round-robin dispatch and padded with -1.  Fewer
+// systems keep the plain order.
 std::vector<int> xcd_order(const npbcg_batch *b, const std::vector<int> &sys_tile,
                            const std::vector<long long> &sys_nnz) {
     const int tiles = sys_tile[b->nsys];
@@ -124,9 +161,69 @@ std::vector<int> xcd_order(const npbcg_batch *b, const std::vector<int> &sys_til
     return out;
 }
 
+// Uploads a workgroup -> unit order into *buf (grown as needed).
+int upload_order(npbcg_batch *b, const std::vector<int> &order, int **buf, size_t *cap) {
+    if (order.size() > *cap) {
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+        B_HIP(hipMalloc(buf, sizeof(int) * order.size()));
+        *cap = order.size();
+    }
+    B_HIP(hipMemcpyAsync(*buf, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice,
+                         b->stream));
+    B_HIP(hipStreamSynchronize(b->stream));
+    return NPBCG_OK;
+}
+
+// Rows per tile of the vector kernels under the SELL layout.
+constexpr int kSellTileRows = 256;
+
+// SELL items (system, block, first slice, end slice) of g windows each, in
+// system order, and the first item of each system.
+void make_sell_items(const SellHost &h, int nsys, int g, std::vector<int> *items,
+                     std::vector<int> *sys_item) {
+    items->clear();
+    sys_item->assign(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+        (*sys_item)[s] = (int)(items->size() / 4);
+        const int nw = h.nrange[s];
+        const std::vector<int> &wg = h.wg[s];
+        for (int blk = 0; blk < h.nb[s]; blk++)
+            for (int r = 0; r < nw; r += g) {
+                items->push_back(s);
+                items->push_back(blk);
+                items->push_back(wg[(size_t)blk * nw + r]);
+                items->push_back(wg[(size_t)blk * nw + std::min(nw, r + g)]);
+            }
+    }
+    (*sys_item)[nsys] = (int)(items->size() / 4);
+}
+
 int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
+    if (layout == 3 && !b->sell_ok)
+        return set_error(NPBCG_ERR_CAPACITY, "batch: no SELL copy (more than 2^31 stored entries)");
     std::vector<int> ts, tr, st;
-    make_tiles(b, rt, ts, tr, st);
+    make_tiles(b, layout == 3 ? kSellTileRows : rt, ts, tr, st);
+    if (layout == 3) {  // items: (system, block, slices of g windows) in system order
+        const int g = std::max(1, rt / kSellWindow);
+        rt = g * kSellWindow;
+        std::vector<int> items, sys_item;
+        make_sell_items(b->sell_h, b->nsys, g, &items, &sys_item);
+        const int ni = sys_item[b->nsys];
+        if (ni > b->items_cap) return set_error(NPBCG_ERR_INTERNAL, "batch: item capacity");
+        B_HIP(hipMemcpyAsync(b->d_items, items.data(), sizeof(int) * items.size(),
+                             hipMemcpyHostToDevice, b->stream));
+        B_HIP(hipMemcpyAsync(b->d_sysinfo, sys_item.data(), sizeof(int) * (b->nsys + 1),
+                             hipMemcpyHostToDevice, b->stream));
+        B_HIP(hipStreamSynchronize(b->stream));
+        const std::vector<int> bi = xcd_order(b, sys_item, b->sys_nnz);
+        int rc = upload_order(b, bi, &b->d_block_item, &b->block_item_cap);
+        if (rc != NPBCG_OK) return rc;
+        b->bd.items = ni;
+        b->bd.item_blocks = (int)bi.size();
+        b->bd.block_item = b->d_block_item;
+    }
     if (layout == 2) {  // items: (system, column block, row range) in system order
         std::vector<int> items, sys_item(b->nsys + 1, 0);
         for (int s = 0; s < b->nsys; s++) {
@@ -149,7 +246,8 @@ int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
         B_HIP(hipStreamSynchronize(b->stream));
         b->bd.items = ni;
     }
-    b->bd.cb = layout == 2 ? 1 : 0;
+    b->bd.cb = layout >= 2 ? 1 : 0;
+    b->bd.sell = layout == 3 ? 1 : 0;
     b->opts.layout = layout;
     const int tiles = (int)ts.size();
     if (tiles > b->tiles_cap) return set_error(NPBCG_ERR_INTERNAL, "batch: tile capacity");
@@ -162,16 +260,8 @@ int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
                          b->stream));
     B_HIP(hipStreamSynchronize(b->stream));
     const std::vector<int> bt = xcd_order(b, st, b->sys_nnz);
-    if (bt.size() > b->block_tile_cap) {
-        (void)hipFree(b->d_block_tile);
-        b->d_block_tile = nullptr;
-        b->block_tile_cap = 0;
-        B_HIP(hipMalloc(&b->d_block_tile, sizeof(int) * bt.size()));
-        b->block_tile_cap = bt.size();
-    }
-    B_HIP(hipMemcpyAsync(b->d_block_tile, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice,
-                         b->stream));
-    B_HIP(hipStreamSynchronize(b->stream));
+    int rc = upload_order(b, bt, &b->d_block_tile, &b->block_tile_cap);
+    if (rc != NPBCG_OK) return rc;
     b->bd.blocks = (int)bt.size();
     b->bd.block_tile = b->d_block_tile;
     b->bd.tiles = tiles;
@@ -184,17 +274,80 @@ int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
     return NPBCG_OK;
 }
 
-// Sizes of the column-blocked layout for the batch's inner type: blocks of
-// cbw columns whose nrhs-wide vector entries fill lds bytes.
+// Sizes of the column-blocked layouts for the batch's inner type: blocks of
+// cbw columns whose nrhs-wide vector entries fill 128 KB of LDS (one SELL
+// workgroup per CU).  Wider blocks than half that measured faster for the
+// SELL SpMM: half as many row partials, twice as long row segments.
 int cb_width(const npbcg_batch *b, int device) {
     int optin = 0;
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) !=
             hipSuccess || optin <= 0)
         optin = 64 * 1024;
     const int elem = b->opts.mode == NPBCG_SOLVE_MIXED ? 4 : 8;
-    const int want = b->opts.mode == NPBCG_SOLVE_MIXED ? 64 * 1024 : 128 * 1024;
+    const int want = 128 * 1024;
     const int lds = std::min(want, optin - 2048);
     return std::max(64, lds / (elem * b->nr));
+}
+
+// SELL copy of every system (local CSR with ascending columns per row, as
+// in the global CSR), concatenated: slice offsets shifted by the entries
+// before the system, lane rows made global.  False if the stored entries
+// exceed 32-bit offsets.
+template <class V>
+bool build_batch_sell(const npbcg_batch *b, int W, const std::vector<int> &rp,
+                      const std::vector<int> &col, const std::vector<double> &val, SellHost *h,
+                      std::vector<V> *sval, std::vector<uint16_t> *scol, std::vector<int> *soff,
+                      std::vector<int> *srow) {
+    long long entries = 0;
+    int slices = 0;
+    h->W = W;
+    h->nb.assign(b->nsys, 0);
+    h->wg.assign(b->nsys, {});
+    h->nrange.assign(b->nsys, 0);
+    h->part.assign(b->nsys, 0);
+    h->part_total = 0;
+    for (int s = 0; s < b->nsys; s++) {
+        h->nb[s] = (b->n[s] + W - 1) / W;
+        h->part[s] = h->part_total;
+        h->part_total += (long long)b->n[s] * h->nb[s] * b->nr;
+    }
+    std::vector<int> lrs, lci;
+    for (int s = 0; s < b->nsys; s++) {
+        const int off = b->row_off[s], ns = b->n[s];
+        const int e0 = rp[off], nnz_s = rp[off + ns] - e0;
+        lrs.resize((size_t)ns + 1);
+        lci.resize((size_t)nnz_s);
+        for (int j = 0; j <= ns; j++) lrs[j] = rp[off + j] - e0;
+        for (int k = 0; k < nnz_s; k++) lci[k] = col[(size_t)e0 + k] - off;
+        SellBlocks<V> sb;
+        if (!build_sell_blocks(ns, lrs.data(), lci.data(), val.data() + e0, W, kSellWindow, &sb) ||
+            sb.nblk != h->nb[s] || entries + sb.stored > (long long)INT32_MAX)
+            return false;
+        h->nrange[s] = sb.nrange;
+        std::vector<int> &wg = h->wg[s];
+        wg.resize(sb.wg_slice.size());
+        for (size_t i = 0; i < wg.size(); i++) wg[i] = slices + sb.wg_slice[i];
+        const int nsl = (int)sb.slice_off.size() - 1;
+        for (int i = 0; i < nsl; i++) soff->push_back((int)(entries + sb.slice_off[i]));
+        for (int r : sb.slice_row) srow->push_back(r >= 0 ? r + off : -1);
+        sval->insert(sval->end(), sb.val.begin(), sb.val.end());
+        scol->insert(scol->end(), sb.col.begin(), sb.col.end());
+        entries += sb.stored;
+        slices += nsl;
+    }
+    soff->push_back((int)entries);
+    return true;
+}
+
+// Column block width of the FP64 residual copy: nrhs-wide FP64 vector
+// entries in 128 KB of LDS.
+int res_width(const npbcg_batch *b, int device) {
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) !=
+            hipSuccess || optin <= 0)
+        optin = 64 * 1024;
+    const int lds = std::min(128 * 1024, optin - 2048);
+    return std::max(64, lds / (8 * b->nr));
 }
 
 template <typename T>
@@ -204,6 +357,7 @@ BatchVecs<T> inner_vecs(const npbcg_batch *b) {
     T *base = static_cast<T *>(b->d_inner);
     v.val = std::is_same<T, float>::value ? (const T *)(const void *)b->d_val32
                                           : (const T *)(const void *)b->d_val;
+    v.sval = static_cast<const T *>(b->d_sell_val);
     v.d = base;
     v.r = base + m;
     v.p = base + 2 * m;
@@ -217,9 +371,43 @@ BatchVecs<T> inner_vecs(const npbcg_batch *b) {
     return v;
 }
 
+// The batch as seen by the FP64 SELL residual: its own column blocks,
+// items and partial offsets; rows, tiles and flags of the batch.
+BatchDev resid_dev(const npbcg_batch *b) {
+    BatchDev d = b->bd;
+    const int ns = b->nsys;
+    d.cb = 1;
+    d.sell = 1;
+    d.cbw = b->res_h.W;
+    d.items = b->res_items;
+    d.item = b->d_res_items;
+    d.sys_item = b->d_res_sys;
+    d.sys_row = b->d_res_sys + ns + 1;
+    d.sys_nb = b->d_res_sys + 2 * ns + 2;
+    d.sys_part = b->d_res_part;
+    d.item_blocks = b->res_item_blocks;
+    d.block_item = b->d_res_block_item;
+    d.sell_off = b->d_res_off;
+    d.sell_row = b->d_res_row;
+    d.sell_col = b->d_res_col;
+    return d;
+}
+
+// res = b - A x (FP64), |res|^2 per solve into d_sysout.
+int true_residual(npbcg_batch *b) {
+    if (b->res_ok)
+        B_HIP(batch_residual_sell(resid_dev(b), b->d_res_val, b->d_b, b->d_x, b->d_res_rows_part,
+                                  b->d_res, b->d_part, b->d_sysout, b->stream));
+    else
+        B_HIP(batch_residual(b->bd, b->d_val, b->d_b, b->d_x, b->d_res, b->d_part, b->d_sysout,
+                             b->stream));
+    return NPBCG_OK;
+}
+
 std::string shape_key(const npbcg_batch *b) {
     char buf[256];
-    std::snprintf(buf, sizeof buf, "batch:sys=%d:rows=%d:nnz=%lld:nrhs=%d:mode=%d", b->nsys,
+    // "batch3": records of the three-layout tuner (older records are not reused)
+    std::snprintf(buf, sizeof buf, "batch3:sys=%d:rows=%d:nnz=%lld:nrhs=%d:mode=%d", b->nsys,
                   b->rows, b->nnz, b->nr, b->opts.mode);
     return buf;
 }
@@ -313,7 +501,16 @@ void npbcg_batch_destroy(npbcg_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    for (void *p : {(void *)b->d_items, (void *)b->d_sysinfo, (void *)b->d_sys64, (void *)b->d_seg,
+    if (b->res_owns)
+        for (void *p : {(void *)b->d_res_val, (void *)b->d_res_col, (void *)b->d_res_off,
+                        (void *)b->d_res_row, (void *)b->d_res_rows_part})
+            (void)hipFree(p);
+    for (void *p : {(void *)b->d_res_items, (void *)b->d_res_sys, (void *)b->d_res_part,
+                    (void *)b->d_res_block_item})
+        (void)hipFree(p);
+    for (void *p : {b->d_sell_val, (void *)b->d_sell_col, (void *)b->d_sell_off,
+                    (void *)b->d_sell_row, (void *)b->d_block_item,
+                    (void *)b->d_items, (void *)b->d_sysinfo, (void *)b->d_sys64, (void *)b->d_seg,
                     b->d_rows_part, (void *)b->d_rowptr, (void *)b->d_col, (void *)b->d_val,
                     (void *)b->d_val32,
                     (void *)b->d_tiles, (void *)b->d_block_tile, (void *)b->d_flags, (void *)b->d_sc, (void *)b->d_b,
@@ -337,7 +534,7 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     if (opts) o = *opts;
     if ((o.nrhs != 1 && o.nrhs != 2 && o.nrhs != 4) ||
         (o.mode != NPBCG_SOLVE_FP64 && o.mode != NPBCG_SOLVE_MIXED) || o.layout < 0 ||
-        o.layout > 2)
+        o.layout > 3)
         return set_error(NPBCG_ERR_INVALID,
                          "npbcg_batch_create: nrhs must be 1, 2 or 4; mode and layout valid");
     long long rows = 0, nnz = 0;
@@ -427,6 +624,31 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
         }
     }
     rp[rows] = (int)e;
+    // SELL copy in the inner type (layout 3)
+    std::vector<float> sval32;
+    std::vector<double> sval64;
+    std::vector<uint16_t> scol;
+    std::vector<int> soff, srow;
+    b->sell_ok = o.mode == NPBCG_SOLVE_MIXED
+                     ? build_batch_sell<float>(b, b->cbw, rp, col, val, &b->sell_h, &sval32, &scol,
+                                               &soff, &srow)
+                     : build_batch_sell<double>(b, b->cbw, rp, col, val, &b->sell_h, &sval64,
+                                                &scol, &soff, &srow);
+    // FP64 copy of the true residual (MIXED: a copy of its own)
+    std::vector<double> rval;
+    std::vector<uint16_t> rcol;
+    std::vector<int> roff, rrow;
+    if (o.mode == NPBCG_SOLVE_MIXED) {
+        b->res_ok = build_batch_sell<double>(b, res_width(b, b->device), rp, col, val, &b->res_h,
+                                             &rval, &rcol, &roff, &rrow);
+        b->res_owns = b->res_ok;
+    } else {
+        b->res_ok = b->sell_ok;
+        b->res_h = b->sell_h;
+    }
+    if (o.layout == 3 && !b->sell_ok)
+        return fail(set_error(NPBCG_ERR_CAPACITY,
+                              "npbcg_batch_create: SELL copy exceeds 32-bit offsets"));
     const size_t m = (size_t)rows * b->nr;
     b->tiles_cap = max_tiles(b, kMinTile);
     b->items_cap = 0;
@@ -466,6 +688,76 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     TRY(hipMemcpyAsync(b->d_seg, seg.data(), sizeof(int) * seg_total, hipMemcpyHostToDevice,
                        b->stream));
     TRY(hipMalloc(&b->d_sysout, sizeof(double) * nsys * b->nr));
+    if (b->sell_ok) {
+        const size_t stored = scol.size();
+        const size_t vbytes = o.mode == NPBCG_SOLVE_MIXED ? sizeof(float) * stored
+                                                          : sizeof(double) * stored;
+        TRY(hipMalloc(&b->d_sell_val, std::max<size_t>(vbytes, 8)));
+        TRY(hipMalloc(&b->d_sell_col, std::max<size_t>(sizeof(uint16_t) * stored, 8)));
+        TRY(hipMalloc(&b->d_sell_off, sizeof(int) * soff.size()));
+        TRY(hipMalloc(&b->d_sell_row, sizeof(int) * std::max<size_t>(srow.size(), 1)));
+        const void *vsrc = o.mode == NPBCG_SOLVE_MIXED ? (const void *)sval32.data()
+                                                       : (const void *)sval64.data();
+        TRY(hipMemcpyAsync(b->d_sell_val, vsrc, vbytes, hipMemcpyHostToDevice, b->stream));
+        TRY(hipMemcpyAsync(b->d_sell_col, scol.data(), sizeof(uint16_t) * stored,
+                           hipMemcpyHostToDevice, b->stream));
+        TRY(hipMemcpyAsync(b->d_sell_off, soff.data(), sizeof(int) * soff.size(),
+                           hipMemcpyHostToDevice, b->stream));
+        TRY(hipMemcpyAsync(b->d_sell_row, srow.data(), sizeof(int) * srow.size(),
+                           hipMemcpyHostToDevice, b->stream));
+    }
+    if (b->res_ok) {
+        if (b->res_owns) {
+            const size_t stored = rcol.size();
+            TRY(hipMalloc(&b->d_res_val, std::max<size_t>(sizeof(double) * stored, 8)));
+            TRY(hipMalloc(&b->d_res_col, std::max<size_t>(sizeof(uint16_t) * stored, 8)));
+            TRY(hipMalloc(&b->d_res_off, sizeof(int) * roff.size()));
+            TRY(hipMalloc(&b->d_res_row, sizeof(int) * std::max<size_t>(rrow.size(), 1)));
+            TRY(hipMalloc(&b->d_res_rows_part,
+                          sizeof(double) * std::max<long long>(b->res_h.part_total, 1)));
+            TRY(hipMemcpyAsync(b->d_res_val, rval.data(), sizeof(double) * stored,
+                               hipMemcpyHostToDevice, b->stream));
+            TRY(hipMemcpyAsync(b->d_res_col, rcol.data(), sizeof(uint16_t) * stored,
+                               hipMemcpyHostToDevice, b->stream));
+            TRY(hipMemcpyAsync(b->d_res_off, roff.data(), sizeof(int) * roff.size(),
+                               hipMemcpyHostToDevice, b->stream));
+            TRY(hipMemcpyAsync(b->d_res_row, rrow.data(), sizeof(int) * rrow.size(),
+                               hipMemcpyHostToDevice, b->stream));
+        } else {  // FP64 mode: the inner SELL copy and its row partials
+            b->d_res_val = static_cast<double *>(b->d_sell_val);
+            b->d_res_col = b->d_sell_col;
+            b->d_res_off = b->d_sell_off;
+            b->d_res_row = b->d_sell_row;
+            b->d_res_rows_part = static_cast<double *>(b->d_rows_part);
+        }
+        std::vector<int> items, sys_item;
+        make_sell_items(b->res_h, nsys, 1, &items, &sys_item);
+        b->res_items = sys_item[nsys];
+        std::vector<int> sys(3 * (size_t)nsys + 2);
+        for (int s = 0; s <= nsys; s++) {
+            sys[s] = sys_item[s];
+            sys[nsys + 1 + s] = b->row_off[s];
+        }
+        for (int s = 0; s < nsys; s++) sys[2 * nsys + 2 + s] = b->res_h.nb[s];
+        const std::vector<int> order = xcd_order(b, sys_item, b->sys_nnz);
+        b->res_item_blocks = (int)order.size();
+        TRY(hipMalloc(&b->d_res_items, sizeof(int) * std::max<size_t>(items.size(), 4)));
+        TRY(hipMalloc(&b->d_res_sys, sizeof(int) * sys.size()));
+        TRY(hipMalloc(&b->d_res_part, sizeof(long long) * nsys));
+        TRY(hipMalloc(&b->d_res_block_item, sizeof(int) * std::max<size_t>(order.size(), 1)));
+        TRY(hipMemcpyAsync(b->d_res_items, items.data(), sizeof(int) * items.size(),
+                           hipMemcpyHostToDevice, b->stream));
+        TRY(hipMemcpyAsync(b->d_res_sys, sys.data(), sizeof(int) * sys.size(),
+                           hipMemcpyHostToDevice, b->stream));
+        TRY(hipMemcpyAsync(b->d_res_part, b->res_h.part.data(), sizeof(long long) * nsys,
+                           hipMemcpyHostToDevice, b->stream));
+        TRY(hipMemcpyAsync(b->d_res_block_item, order.data(), sizeof(int) * order.size(),
+                           hipMemcpyHostToDevice, b->stream));
+        const hipError_t e3 = batch_prepare_resid(b->res_h.W * b->nr * (int)sizeof(double));
+        if (e3 != hipSuccess)
+            return fail(set_error(NPBCG_ERR_DEVICE, std::string("batch_prepare_resid: ") +
+                                                        hipGetErrorString(e3)));
+    }
     TRY(hipMemcpyAsync(b->d_rowptr, rp.data(), sizeof(int) * (rows + 1), hipMemcpyHostToDevice,
                        b->stream));
     TRY(hipMemcpyAsync(b->d_col, col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, b->stream));
@@ -495,6 +787,9 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     b->bd.sys_seg = b->d_sys64;
     b->bd.sys_part = b->d_sys64 + nsys;
     b->bd.seg = b->d_seg;
+    b->bd.sell_off = b->d_sell_off;
+    b->bd.sell_row = b->d_sell_row;
+    b->bd.sell_col = b->d_sell_col;
     {
         const int lds = b->cbw * b->nr * (o.mode == NPBCG_SOLVE_MIXED ? 4 : 8);
         hipError_t e2 = o.mode == NPBCG_SOLVE_MIXED ? batch_prepare_cb<float>(lds)
@@ -510,7 +805,9 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     if (rt == 0) rt = dr;
     if ((lpr & (lpr - 1)) || lpr < 4 || lpr > 64 || rt < kMinTile || rt > 4096)
         return fail(set_error(NPBCG_ERR_INVALID, "npbcg_batch_create: invalid launch shape"));
-    int rc = apply_shape(b, lpr, rt, o.layout == 1 ? 1 : 2);
+    const int layout = o.layout != 0 ? o.layout : b->sell_ok ? 3 : 2;
+    if (layout == 3 && o.rows_per_tile == 0) rt = kSellWindow;
+    int rc = apply_shape(b, lpr, rt, layout);
     if (rc != NPBCG_OK) return fail(rc);
     *out = b;
     clear_error();
@@ -540,9 +837,15 @@ int npbcg_batch_tune(npbcg_batch *b, const char *cache_path, int *lanes_per_row,
         // whatever they hold (timing only); state is reset afterwards.
         B_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * 2 * b->nsys * b->nr, b->stream));
         best = 1e30;
-        for (int cy : {1, 2})
+        int dl = 0, dr = 0;
+        default_shape(b, &dl, &dr);
+        for (int cy : {1, 2, 3})
             for (int cl : {4, 8, 16, 32, 64})
                 for (int cr : {64, 128, 256, 512, 1024, 2048, 4096}) {
+                    // layout 3: one lane per row, items of whole sorting
+                    // windows; lanes_per_row stays at its default (it shapes
+                    // only the FP64 residual kernel)
+                    if (cy == 3 && (!b->sell_ok || cl != dl || cr < kSellWindow)) continue;
                     int rc = apply_shape(b, cl, cr, cy);
                     if (rc != NPBCG_OK) return rc;
                     double t = 0;
@@ -583,8 +886,7 @@ int npbcg_batch_solve(npbcg_batch *b, double rtol, int maxit, npbcg_batch_stats 
     B_HIP(hipMemsetAsync(b->d_x, 0, sizeof(double) * (size_t)b->rows * b->nr, b->stream));
     B_HIP(hipMemsetAsync(b->d_sc, 0, sizeof(BatchScalars) * ns, b->stream));
     B_HIP(hipEventRecord(b->ev0, b->stream));
-    B_HIP(batch_residual(b->bd, b->d_val, b->d_b, b->d_x, b->d_res, b->d_part, b->d_sysout,
-                         b->stream));
+    B_HIP(batch_residual_zero(b->bd, b->d_b, b->d_res, b->d_part, b->d_sysout, b->stream));
     B_HIP(hipMemcpyAsync(bb.data(), b->d_sysout, sizeof(double) * ns, hipMemcpyDeviceToHost,
                          b->stream));
     B_HIP(hipStreamSynchronize(b->stream));
@@ -619,8 +921,8 @@ int npbcg_batch_solve(npbcg_batch *b, double rtol, int maxit, npbcg_batch_stats 
                              b->stream));
         int rc = mixed ? run_inner<float>(b, b->bd.maxit) : run_inner<double>(b, b->bd.maxit);
         if (rc != NPBCG_OK) return rc;
-        B_HIP(batch_residual(b->bd, b->d_val, b->d_b, b->d_x, b->d_res, b->d_part, b->d_sysout,
-                             b->stream));
+        rc = true_residual(b);
+        if (rc != NPBCG_OK) return rc;
         B_HIP(hipMemcpyAsync(rr.data(), b->d_sysout, sizeof(double) * ns, hipMemcpyDeviceToHost,
                              b->stream));
         B_HIP(hipStreamSynchronize(b->stream));

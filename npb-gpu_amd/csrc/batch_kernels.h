@@ -5,7 +5,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace npbcg_dev {
+
+// Rows of a SELL sorting window (layout 3): a system's rows are sorted by
+// their segment length inside windows of this many rows.
+constexpr int kSellWindow = 2048;
 
 // Per (system, right-hand side) scalars of the inner CG.
 struct BatchScalars {
@@ -49,6 +55,17 @@ struct BatchDev {
     const long long *sys_part = nullptr;  // [nsys] start of the system's row partials
     const int *seg = nullptr;       // per row, nb + 1 entry offsets: block b's entries
                                     // are [seg[b], seg[b+1]) (columns sorted in a row)
+    // SELL SpMM (sell = 1, also cb = 1): a column-blocked SELL copy of every
+    // system (blocks of cbw columns, rows sorted by segment length inside
+    // kSellWindow-row windows, 64-row slices stored column-major, 16-bit
+    // columns inside the block).  Item = (system, block, first slice, end
+    // slice); the row partials and p.q partials are laid out as for cb.
+    int sell = 0;
+    int item_blocks = 0;               // workgroups of the SELL SpMM
+    const int *block_item = nullptr;   // [item_blocks] item of each workgroup, -1: none
+    const int *sell_off = nullptr;     // [slices + 1] first entry of each slice
+    const int *sell_row = nullptr;     // [slices * 64] global row of each lane, -1: padding
+    const uint16_t *sell_col = nullptr;  // [stored] column inside the block
     int *done = nullptr;            // [nsys * nr] right-hand side finished
     int *done_next = nullptr;       // flags set during a step, published after it
     BatchScalars *sc = nullptr;     // [nsys * nr]
@@ -58,6 +75,7 @@ struct BatchDev {
 template <typename T>
 struct BatchVecs {
     const T *val = nullptr;   // matrix values in type T
+    const T *sval = nullptr;  // SELL copy's values in type T (layout 3)
     T *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
     T *rows_part = nullptr;   // column-blocked SpMM: per-block row partials
     double *part_init = nullptr, *part_pq = nullptr, *part_rr[2] = {nullptr, nullptr};
@@ -67,9 +85,15 @@ struct BatchVecs {
 template <typename T>
 hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, int first,
                          hipStream_t st);
-// Allows `bytes` of dynamic LDS for the column-blocked SpMM kernels of type T.
+// Allows `bytes` of dynamic LDS for the column-blocked and SELL SpMM kernels
+// of type T.
 template <typename T>
 hipError_t batch_prepare_cb(int bytes);
+// Allows `bytes` of dynamic LDS for the FP64 SELL residual kernels.
+hipError_t batch_prepare_resid(int bytes);
+// Workgroup size of the SELL SpMM: one 1024-thread workgroup per CU, which
+// holds a column block of up to 128 KB in LDS.
+constexpr int kSellBlock = 1024;
 // q = A p and p.q partials only (the SpMM the autotuner times).
 template <typename T>
 hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st);
@@ -82,6 +106,15 @@ hipError_t batch_refine(const BatchDev &bd, double *x, const T *d, hipStream_t s
 // res = b - A x (FP64) and |res|^2 per (system, rhs) into sys_out[nsys * nr].
 hipError_t batch_residual(const BatchDev &bd, const double *val, const double *b, const double *x,
                           double *res, double *part, double *sys_out, hipStream_t st);
+// The same from an FP64 SELL copy described by bd (its items, column blocks
+// and partial offsets): row partials of A x into rows_part (FP64), then the
+// combine.
+hipError_t batch_residual_sell(const BatchDev &bd, const double *sval, const double *b,
+                               const double *x, double *rows_part, double *res, double *part,
+                               double *sys_out, hipStream_t st);
+// Residual of x = 0 (res = b).
+hipError_t batch_residual_zero(const BatchDev &bd, const double *b, double *res, double *part,
+                               double *sys_out, hipStream_t st);
 hipError_t batch_narrow(const double *in, float *out, long long count, hipStream_t st);
 
 }  // namespace npbcg_dev

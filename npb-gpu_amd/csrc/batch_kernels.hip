@@ -52,9 +52,25 @@ __device__ __forceinline__ VecT<T, NR> load_vec(const T *__restrict__ p) {
     return r;
 }
 
+// LDS copy of one vector row (NR values), as one store where it is 16 bytes.
+template <typename T, int NR>
+__device__ __forceinline__ void store_vec(T *p, const VecT<T, NR> &v) {
+    if constexpr (sizeof(T) * NR == 16) {
+        typedef __attribute__((__vector_size__(16))) T v16;
+        v16 x;
+#pragma unroll
+        for (int k = 0; k < NR; k++) x[k] = v.v[k];
+        *reinterpret_cast<v16 *>(p) = x;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NR; k++) p[k] = v.v[k];
+    }
+}
+
 // Block sums of NR per-thread doubles; thread 0 writes out[0..NR).
-template <int NR>
+template <int NR, int BLOCK = kBlock>
 __device__ __forceinline__ void block_sum_store(const double (&acc)[NR], double *out) {
+    constexpr int kWavesPerBlock = BLOCK / kWave;
     __shared__ double ws[NR][kWavesPerBlock];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
@@ -256,6 +272,202 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_cb_kernel(BatchDev bd,
         }
     }
     block_sum_store<NR>(dot, pq_part + (size_t)it * NR);
+}
+
+// SELL SpMM: item (system s, column block b, slices [sl0, sl1)).  The
+// block's vector entries (cbw rows of p, NR values each) are staged in LDS;
+// each wavefront then streams whole slices, one row per lane, so the (value,
+// 16-bit column) loads are contiguous across the wavefront and a row needs no
+// cross-lane reduction.  The row's block sum goes to the block's row partials
+// (added up by the update kernel); p.q partials follow by linearity.  Slices
+// go to wavefronts in a fixed snake order, so the sums are reproducible.
+//
+// The entry stream is software-pipelined in chunks of UNR entries: the loads
+// of the next chunk (or of the next slice's first chunk and its lane rows)
+// are issued before the current chunk's LDS gathers and FMAs, and the first
+// chunk is issued before the LDS fill.  A chunk past the slice's width
+// re-reads the slice's last entry with value 0, so every load stays inside the
+// slice and no tail runs one dependent load at a time.
+template <typename T, int UNR>
+__device__ __forceinline__ void sell_chunk(const T *__restrict__ sval,
+                                           const uint16_t *__restrict__ col, int base, int width,
+                                           int j0, int lane, T (&a)[UNR], int (&c)[UNR]) {
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        const int j = min(j0 + u, width - 1);
+        const size_t at = (size_t)base + (size_t)j * kWave + lane;
+        const T v = sval[at];
+        c[u] = col[at];
+        a[u] = j0 + u < width ? v : T(0);
+    }
+}
+
+//
+// RESID = true computes the row partials of A x for the FP64 true residual:
+// every system, no p.q partials.
+template <typename T, int NR, int BLOCK, int UNR, bool RESID>
+__global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
+                                                                const T *__restrict__ sval,
+                                                                const T *__restrict__ p,
+                                                                T *__restrict__ rows_part,
+                                                                double *__restrict__ pq_part) {
+    extern __shared__ double2 lds_sell[];  // 16-byte aligned
+    T *vlds = reinterpret_cast<T *>(lds_sell);
+    constexpr int kWaves = BLOCK / kWave;
+    const int it = bd.block_item[blockIdx.x];
+    if (it < 0) return;
+    const int s = bd.item[4 * it];
+    if (!RESID && system_done<NR>(bd.done, s)) return;
+    const int b = bd.item[4 * it + 1], sl0 = bd.item[4 * it + 2];
+    const int nsl = bd.item[4 * it + 3] - sl0;
+    const int srow = bd.sys_row[s], ns = bd.sys_row[s + 1] - srow;
+    const int c0 = b * bd.cbw;
+    const int w = min(bd.cbw, ns - c0);
+    const int lane = threadIdx.x & (kWave - 1);
+    // wave-uniform (scalar) index: every branch on the slice loop is scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int *__restrict__ off = bd.sell_off;
+    const uint16_t *__restrict__ col = bd.sell_col;
+    const int *__restrict__ lane_row = bd.sell_row;
+    // Slice of round rd for this wave (snake order: slices of a window are
+    // sorted widest first, and alternating the direction of each round evens
+    // out the width per wavefront), or -1.  Rounds are increasing in k, so
+    // after the first -1 there is none.
+    auto slice_of = [&](int rd) {
+        const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
+        return k < nsl ? sl0 + k : -1;
+    };
+    T na[UNR];
+    int nc[UNR];
+    int cur = slice_of(0), cbase = 0, cwidth = 0, nrow = -1;
+    if (cur >= 0) {
+        cbase = off[cur];
+        cwidth = (off[cur + 1] - cbase) / kWave;
+        nrow = lane_row[(size_t)cur * kWave + lane];
+        if (cwidth > 0) sell_chunk<T, UNR>(sval, col, cbase, cwidth, 0, lane, na, nc);
+    }
+    {
+        const T *src = p + (size_t)(srow + c0) * NR;
+        for (int i = threadIdx.x; i < w; i += BLOCK)
+            store_vec<T, NR>(vlds + i * NR, load_vec<T, NR>(src + (size_t)i * NR));
+    }
+    __syncthreads();
+    T *part = rows_part + bd.sys_part[s] + (size_t)b * ns * NR;
+    double dot[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) dot[q] = 0.0;
+    for (int rd = 0; cur >= 0; rd++) {
+        const int width = cwidth, base = cbase, row = nrow;
+        const int nxt = slice_of(rd + 1);
+        int xbase = 0, xwidth = 0;
+        if (nxt >= 0) {
+            xbase = off[nxt];
+            xwidth = (off[nxt + 1] - xbase) / kWave;
+        }
+        // p of the lane's row, for p.q (a padding lane reads row srow, unused)
+        VecT<T, NR> pv;
+        if constexpr (!RESID) pv = load_vec<T, NR>(p + (size_t)(row >= 0 ? row : srow) * NR);
+        T acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[q] = T(0);
+        const int nch = (width + UNR - 1) / UNR;
+        for (int ch = 0; ch < nch; ch++) {
+            T a[UNR];
+            int c[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                a[u] = na[u];
+                c[u] = nc[u];
+            }
+            if (ch + 1 < nch) {
+                sell_chunk<T, UNR>(sval, col, base, width, (ch + 1) * UNR, lane, na, nc);
+            } else if (nxt >= 0) {
+                nrow = lane_row[(size_t)nxt * kWave + lane];
+                if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                const VecT<T, NR> xv = load_vec<T, NR>(vlds + c[u] * NR);
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[q] = fma(a[u], xv.v[q], acc[q]);
+            }
+        }
+        if (nch == 0 && nxt >= 0) {
+            nrow = lane_row[(size_t)nxt * kWave + lane];
+            if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
+        }
+        if (row >= 0) {
+            VecT<T, NR> out;
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+                out.v[q] = acc[q];
+                if constexpr (!RESID) dot[q] += (double)pv.v[q] * (double)acc[q];
+            }
+            store_vec<T, NR>(part + (size_t)(row - srow) * NR, out);
+        }
+        cur = nxt;
+        cbase = xbase;
+        cwidth = xwidth;
+    }
+    if constexpr (!RESID) block_sum_store<NR, BLOCK>(dot, pq_part + (size_t)it * NR);
+}
+
+// FP64 true residual from the SELL row partials of A x: res = b - sum over
+// the system's column blocks (in order) of the partials; |res|^2 partials per
+// tile.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void batch_resid_combine_kernel(BatchDev bd,
+                                                                     const double *__restrict__ rows_part,
+                                                                     const double *__restrict__ b,
+                                                                     double *__restrict__ res,
+                                                                     double *__restrict__ part) {
+    const int t = blockIdx.x;
+    const int s = bd.tile_sys[t];
+    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
+    const int srow = bd.sys_row[s], ns = bd.sys_row[s + 1] - srow, nb = bd.sys_nb[s];
+    const double *sp = rows_part + bd.sys_part[s];
+    double nrm[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) nrm[k] = 0.0;
+    for (int row = row0 + threadIdx.x; row < row1; row += kBlock) {
+        const size_t local = (size_t)(row - srow) * NR;
+        double ax[NR];
+#pragma unroll
+        for (int k = 0; k < NR; k++) ax[k] = sp[local + k];
+        for (int blk = 1; blk < nb; blk++)
+#pragma unroll
+            for (int k = 0; k < NR; k++) ax[k] += sp[(size_t)blk * ns * NR + local + k];
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const size_t at = (size_t)row * NR + k;
+            const double rv = b[at] - ax[k];
+            res[at] = rv;
+            nrm[k] += rv * rv;
+        }
+    }
+    block_sum_store<NR>(nrm, part + (size_t)t * NR);
+}
+
+// Residual of x = 0: res = b; |b|^2 partials per tile.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void batch_resid_zero_kernel(BatchDev bd,
+                                                                  const double *__restrict__ b,
+                                                                  double *__restrict__ res,
+                                                                  double *__restrict__ part) {
+    const int t = blockIdx.x;
+    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
+    double nrm[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) nrm[k] = 0.0;
+    for (int row = row0 + threadIdx.x; row < row1; row += kBlock)
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const size_t at = (size_t)row * NR + k;
+            const double v = b[at];
+            res[at] = v;
+            nrm[k] += v * v;
+        }
+    block_sum_store<NR>(nrm, part + (size_t)t * NR);
 }
 
 // alpha = rho / p.q per RHS of the tile's system; d += alpha p, r -= alpha q;
@@ -508,6 +720,28 @@ hipError_t with_nr(int nr, F &&f) {
     }
 }
 
+// Entries per pipelined chunk of the SELL SpMM: 8, except FP64 with four
+// right-hand sides (1024-thread workgroups leave 128 registers a lane).
+template <typename T, int NR>
+constexpr int sell_unr() {
+    return sizeof(T) == 8 && NR == 4 ? 4 : 8;
+}
+
+// q = A p (row sums, or block row partials for cb) and p.q partials.
+template <typename T, int NR, int LPR>
+void launch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st) {
+    if (bd.sell) {
+        batch_spmm_sell_kernel<T, NR, kSellBlock, sell_unr<T, NR>(), false>
+            <<<bd.item_blocks, kSellBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
+                bd, v.sval, v.p, v.rows_part, v.part_pq);
+    } else if (bd.cb) {
+        batch_spmm_cb_kernel<T, NR, LPR><<<bd.items, kBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
+            bd, v.val, v.p, v.rows_part, v.part_pq);
+    } else {
+        batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q, v.part_pq);
+    }
+}
+
 }  // namespace
 
 template <typename T>
@@ -520,13 +754,7 @@ hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, in
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            if (bd.cb)
-                batch_spmm_cb_kernel<T, NR, LPR>
-                    <<<bd.items, kBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
-                        bd, v.val, v.p, v.rows_part, v.part_pq);
-            else
-                batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
-                                                                            v.part_pq);
+            launch_spmm<T, NR, LPR>(bd, v, st);
             return hipGetLastError();
         });
     });
@@ -552,13 +780,7 @@ hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st)
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            if (bd.cb)
-                batch_spmm_cb_kernel<T, NR, LPR>
-                    <<<bd.items, kBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
-                        bd, v.val, v.p, v.rows_part, v.part_pq);
-            else
-                batch_spmm_kernel<T, NR, LPR><<<bd.blocks, kBlock, 0, st>>>(bd, v.val, v.p, v.q,
-                                                                            v.part_pq);
+            launch_spmm<T, NR, LPR>(bd, v, st);
             return hipGetLastError();
         });
     });
@@ -577,11 +799,28 @@ hipError_t batch_prepare_cb(int bytes) {
     set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 8>));        \
     set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 16>));       \
     set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 32>));       \
-    set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 64>));
+    set(reinterpret_cast<const void *>(&batch_spmm_cb_kernel<T, NR, 64>));        \
+    set(reinterpret_cast<const void *>(                                          \
+        &batch_spmm_sell_kernel<T, NR, kSellBlock, sell_unr<T, NR>(), false>));
     NPBCG_CB_SET(1)
     NPBCG_CB_SET(2)
     NPBCG_CB_SET(4)
 #undef NPBCG_CB_SET
+    return last;
+}
+
+hipError_t batch_prepare_resid(int bytes) {
+    hipError_t last = hipSuccess;
+    auto set = [&](const void *f) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) last = e;
+    };
+    set(reinterpret_cast<const void *>(
+        &batch_spmm_sell_kernel<double, 1, kSellBlock, sell_unr<double, 1>(), true>));
+    set(reinterpret_cast<const void *>(
+        &batch_spmm_sell_kernel<double, 2, kSellBlock, sell_unr<double, 2>(), true>));
+    set(reinterpret_cast<const void *>(
+        &batch_spmm_sell_kernel<double, 4, kSellBlock, sell_unr<double, 4>(), true>));
     return last;
 }
 
@@ -615,6 +854,37 @@ hipError_t batch_residual(const BatchDev &bd, const double *val, const double *b
     if (e != hipSuccess) return e;
     return with_nr(bd.nr, [&](auto nrc) {
         constexpr int NR = decltype(nrc)::value;
+        batch_system_sum_kernel<NR><<<bd.nsys, kBlock, 0, st>>>(bd, part, sys_out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t batch_residual_sell(const BatchDev &bd, const double *sval, const double *b,
+                               const double *x, double *rows_part, double *res, double *part,
+                               double *sys_out, hipStream_t st) {
+    hipError_t e = with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        batch_spmm_sell_kernel<double, NR, kSellBlock, sell_unr<double, NR>(), true>
+            <<<bd.item_blocks, kSellBlock, (size_t)bd.cbw * NR * sizeof(double), st>>>(
+                bd, sval, x, rows_part, nullptr);
+        hipError_t e2 = hipGetLastError();
+        if (e2 != hipSuccess) return e2;
+        batch_resid_combine_kernel<NR><<<bd.tiles, kBlock, 0, st>>>(bd, rows_part, b, res, part);
+        e2 = hipGetLastError();
+        if (e2 != hipSuccess) return e2;
+        batch_system_sum_kernel<NR><<<bd.nsys, kBlock, 0, st>>>(bd, part, sys_out);
+        return hipGetLastError();
+    });
+    return e;
+}
+
+hipError_t batch_residual_zero(const BatchDev &bd, const double *b, double *res, double *part,
+                               double *sys_out, hipStream_t st) {
+    return with_nr(bd.nr, [&](auto nrc) {
+        constexpr int NR = decltype(nrc)::value;
+        batch_resid_zero_kernel<NR><<<bd.tiles, kBlock, 0, st>>>(bd, b, res, part);
+        hipError_t e2 = hipGetLastError();
+        if (e2 != hipSuccess) return e2;
         batch_system_sum_kernel<NR><<<bd.nsys, kBlock, 0, st>>>(bd, part, sys_out);
         return hipGetLastError();
     });

@@ -28,7 +28,8 @@ def _rhs(mats, nrhs):
 
 
 @pytest.mark.parametrize("mode,nrhs,layout", [(1, 4, 2), (0, 4, 2), (1, 1, 2), (0, 2, 2),
-                                              (1, 4, 1), (0, 1, 1)])
+                                              (1, 4, 1), (0, 1, 1), (1, 4, 3), (0, 4, 3),
+                                              (1, 1, 3), (0, 1, 3), (1, 2, 3), (0, 2, 3)])
 def test_batch_solves_match_oracle(gpu, mode, nrhs, layout):
     mats = _systems(gpu)
     rhs = _rhs(mats, nrhs)
@@ -37,6 +38,7 @@ def test_batch_solves_match_oracle(gpu, mode, nrhs, layout):
         st, steps, rel = bt.solve(RTOL, 3000)
         xs = bt.solution()
     print(f"mode {mode} nrhs {nrhs}: {st.as_dict()}; steps {steps.ravel().tolist()}")
+    assert st.layout == layout
     assert st.converged == len(mats) * nrhs and st.max_relres <= RTOL
     for s, m in enumerate(mats):
         om = _ocsr(m)
@@ -56,12 +58,14 @@ def test_batch_launch_shapes_agree(gpu):
     rhs = _rhs(mats, 4)
     sols = []
     for lpr, rt, layout in [(4, 64, 1), (16, 256, 1), (64, 1024, 1), (8, 64, 2), (16, 512, 2),
-                            (64, 4096, 2)]:
+                            (64, 4096, 2), (8, 512, 3), (16, 3000, 3), (16, 4096, 3)]:
         with gpu.Batch(mats, nrhs=4, mode=0, lanes_per_row=lpr, rows_per_tile=rt,
                        layout=layout) as bt:
             bt.set_rhs(rhs)
             st, _, _ = bt.solve(RTOL, 3000)
             assert st.layout == layout and st.converged == st.solves
+            if layout == 3:  # whole 2048-row sorting windows per workgroup
+                assert st.rows_per_tile == max(2048, rt // 2048 * 2048)
             sols.append(np.concatenate(bt.solution()))
     for x in sols[1:]:
         assert np.max(np.abs(x - sols[0])) <= 1e-9 * np.max(np.abs(sols[0]))
@@ -104,9 +108,31 @@ def test_batch_many_systems_xcd_placement(gpu):
         bt.set_rhs(rhs)
         st, steps, rel = bt.solve(RTOL, 3000)
         xs = bt.solution()
+    assert st.layout == 3  # the default layout: SELL slices
     assert st.converged == 48 and st.max_relres <= RTOL
     for s, m in enumerate(mats):
         om = _ocsr(m)
         for k in range(4):
             b = rhs[s][:, k]
             assert np.linalg.norm(b - oracle.spmv(om, xs[s][:, k])) <= RTOL * np.linalg.norm(b)
+
+
+def test_batch_sell_many_blocks_and_windows(gpu):
+    # systems wide enough for several column blocks (8192 columns with four
+    # FP32 right-hand sides, 4096 with four FP64 ones) and several 2048-row
+    # sorting windows per block, checked against the oracle's CG
+    params = [(9000, 9, 20.0, 7.0), (20000, 10, 20.0, 8.0), (33000, 8, 20.0, 9.0)]
+    mats = gpu.make_many(params, workers=3)
+    rhs = _rhs(mats, 4)
+    for mode in (1, 0):
+        with gpu.Batch(mats, nrhs=4, mode=mode, layout=3, rows_per_tile=4096) as bt:
+            bt.set_rhs(rhs)
+            st, steps, rel = bt.solve(RTOL, 3000)
+            xs = bt.solution()
+        assert st.layout == 3 and st.converged == 12 and st.max_relres <= RTOL
+        for s, m in enumerate(mats):
+            om = _ocsr(m)
+            for k in (0, 3):
+                b = rhs[s][:, k]
+                ox, _, _ = oracle.cg_solve(om, b, rtol=RTOL, maxit=5000)
+                assert np.max(np.abs(xs[s][:, k] - ox)) <= 1e-8 * np.max(np.abs(ox))
