@@ -282,29 +282,10 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_cb_kernel(BatchDev bd,
 // (added up by the update kernel); p.q partials follow by linearity.  Slices
 // go to wavefronts in a fixed snake order, so the sums are reproducible.
 //
-// The entry stream is software-pipelined in chunks of UNR entries: the loads
-// of the next chunk (or of the next slice's first chunk and its lane rows)
-// are issued before the current chunk's LDS gathers and FMAs, and the first
-// chunk is issued before the LDS fill.  A chunk past the slice's width
-// re-reads the slice's last entry with value 0, so every load stays inside the
-// slice and no tail runs one dependent load at a time.
-template <typename T, int UNR>
-__device__ __forceinline__ void sell_chunk(const T *__restrict__ sval,
-                                           const uint16_t *__restrict__ col, int base, int width,
-                                           int j0, int lane, T (&a)[UNR], int (&c)[UNR]) {
-#pragma unroll
-    for (int u = 0; u < UNR; u++) {
-        const int j = min(j0 + u, width - 1);
-        const size_t at = (size_t)base + (size_t)j * kWave + lane;
-        const T v = sval[at];
-        c[u] = col[at];
-        a[u] = j0 + u < width ? v : T(0);
-    }
-}
-
-//
-// RESID = true computes the row partials of A x for the FP64 true residual:
-// every system, no p.q partials.
+// The entry stream is software-pipelined with sell_chunk: the loads of the
+// next chunk (or of the next slice's first chunk and its lane rows) are issued
+// before the current chunk's LDS gathers and FMAs, and the first chunk is
+// issued before the LDS fill.
 template <typename T, int NR, int BLOCK, int UNR, bool RESID>
 __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
                                                                 const T *__restrict__ sval,

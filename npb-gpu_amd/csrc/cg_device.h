@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace npbcg_dev {
 
 constexpr int kWave = 64;
@@ -40,6 +42,25 @@ __device__ __forceinline__ double segment_sum(double v) {
     if constexpr (SEG == 64)
         v = (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
     return v;
+}
+
+// One chunk of a SELL slice (entries j0 .. j0+UNR-1 of the lane's row): the
+// slice stores entry j of lane l at base + 64 j + l.  Entries past the slice
+// width re-read the slice's last entry with value 0, so every load stays
+// inside the slice, a chunk's loads can all be in flight together, and the
+// sum gains only exact zeros (a * x + 0 * y).  width must be at least 1.
+template <typename T, int UNR>
+__device__ __forceinline__ void sell_chunk(const T *__restrict__ val,
+                                           const uint16_t *__restrict__ col, int base, int width,
+                                           int j0, int lane, T (&a)[UNR], int (&c)[UNR]) {
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        const int j = min(j0 + u, width - 1);
+        const size_t at = (size_t)base + (size_t)j * kWave + lane;
+        const T v = val[at];
+        c[u] = col[at];
+        a[u] = j0 + u < width ? v : T(0);
+    }
 }
 
 // Device-resident CG scalars of one context.  Kernels never pass values to

@@ -172,20 +172,27 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int s0 = wg_slice[wg];
     const int nsl = wg_slice[wg + 1] - s0;
-    // The first UNR matrix entries of this wave's first slice are loaded
-    // before the vector block is staged, so their HBM latency overlaps the
-    // LDS fill (they do not depend on it).
-    double pre_a[UNR];
-    int pre_c[UNR];
-    const bool pre = wave < nsl &&
-                     (slice_off[s0 + wave + 1] - slice_off[s0 + wave]) / kWave >= UNR;
-    if (pre) {
-        const int base = slice_off[s0 + wave];
-#pragma unroll
-        for (int u = 0; u < UNR; u++) {
-            pre_a[u] = val[base + lane + (size_t)u * kWave];
-            pre_c[u] = col[base + lane + (size_t)u * kWave];
-        }
+    // Slice (index in this workgroup) of round rd for this wave, or -1.
+    // Slices are sorted widest first; alternating the direction of each round
+    // ("snake" order) evens out the total width per wavefront.  The index
+    // grows with rd, so after the first -1 there is none.
+    auto slice_of = [&](int rd) {
+        const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
+        return k < nsl ? k : -1;
+    };
+    // The entry stream is software-pipelined in chunks of UNR entries
+    // (sell_chunk): the next chunk's loads, or the next slice's first chunk
+    // and lane rows, go out before the current chunk's LDS gathers.  The first
+    // chunk of this wave's first slice is loaded before the vector block is
+    // staged, so its memory latency overlaps the LDS fill.
+    double na[UNR];
+    int nc[UNR];
+    int cur = slice_of(0), cbase = 0, cwidth = 0, nrow = -1;
+    if (cur >= 0) {
+        cbase = slice_off[s0 + cur];
+        cwidth = (slice_off[s0 + cur + 1] - cbase) / kWave;
+        nrow = slice_row[(size_t)(s0 + cur) * kWave + lane];
+        if (cwidth > 0) sell_chunk<double, UNR>(val, col, cbase, cwidth, 0, lane, na, nc);
     }
     if constexpr (!PUPD) {
         const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
@@ -195,44 +202,49 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     }
     __syncthreads();
     double *part_b = rows_part + (size_t)b * n;
-    const int rounds = (nsl + kWaves - 1) / kWaves;
-    for (int rd = 0; rd < rounds; rd++) {
-        // Slices are sorted widest first; alternating the direction of each
-        // round ("snake" order) evens out the total width per wavefront.
-        const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
-        if (k >= nsl) continue;
-        const int s = s0 + k;
-        const int base = slice_off[s];
-        const int width = (slice_off[s + 1] - base) / kWave;
-        const double *vp = val + base + lane;
-        const uint16_t *cp = col + base + lane;
-        double acc = 0.0;
-        int j = 0;
-        if (rd == 0 && pre) {
-#pragma unroll
-            for (int u = 0; u < UNR; u++) acc = fma(pre_a[u], vblk[pre_c[u]], acc);
-            j = UNR;
+    for (int rd = 0; cur >= 0; rd++) {
+        const int k = cur, width = cwidth, base = cbase, row = nrow;
+        const int nxt = slice_of(rd + 1);
+        int xbase = 0, xwidth = 0;
+        if (nxt >= 0) {
+            xbase = slice_off[s0 + nxt];
+            xwidth = (slice_off[s0 + nxt + 1] - xbase) / kWave;
         }
-        for (; j + UNR <= width; j += UNR) {
+        // p of the lane's row for p.q, loaded while the slice streams
+        double pr = 0.0;
+        if constexpr (MODE == 0)
+            if (row >= 0) pr = PUPD ? r[row] + beta * v[row] : v[row];
+        double acc = 0.0;
+        const int nch = (width + UNR - 1) / UNR;
+        for (int ch = 0; ch < nch; ch++) {
             double av[UNR];
             int c[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; u++) {
-                av[u] = vp[(size_t)(j + u) * kWave];
-                c[u] = cp[(size_t)(j + u) * kWave];
+                av[u] = na[u];
+                c[u] = nc[u];
+            }
+            if (ch + 1 < nch) {
+                sell_chunk<double, UNR>(val, col, base, width, (ch + 1) * UNR, lane, na, nc);
+            } else if (nxt >= 0) {
+                nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
+                if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
             }
 #pragma unroll
             for (int u = 0; u < UNR; u++) acc = fma(av[u], vblk[c[u]], acc);
         }
-        for (; j < width; j++) acc = fma(vp[(size_t)j * kWave], vblk[cp[(size_t)j * kWave]], acc);
-        const int row = slice_row[(size_t)s * kWave + lane];
+        if (nch == 0 && nxt >= 0) {
+            nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
+            if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
+        }
         if (row >= 0) part_b[row] = acc;
         if constexpr (MODE == 0) {
-            double pr = 0.0;
-            if (row >= 0) pr = PUPD ? r[row] + beta * v[row] : v[row];
             const double d = segment_sum<64>(pr * acc);
             if (lane == 0) slice_dot[k] = d;
         }
+        cur = nxt;
+        cbase = xbase;
+        cwidth = xwidth;
     }
     if constexpr (MODE == 0) {
         __syncthreads();
