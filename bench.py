@@ -138,6 +138,20 @@ def profile_pass():
                           "kernel_config": ctx.describe()}))
 
 
+def profile_pass_batch():
+    """Short run for counter passes on the batched SpMM: the 64-system batch
+    (MIXED, layout 3, 2048-row items) and 20 SpMM launches, every system
+    running."""
+    import npbcg
+
+    bw = BATCH_WORKLOAD
+    mats = npbcg.make_many(batch_systems(bw))
+    with npbcg.Batch(mats, nrhs=bw["nrhs"], mode=npbcg.SOLVE_MIXED, rows_per_tile=2048,
+                     layout=3) as bt:
+        ms = bt.spmm_bench(20)
+    print(json.dumps({"profile_pass_batch": True, "spmm_ms": round(ms, 4)}))
+
+
 def _pmc_rows(path, counter):
     import csv
 
@@ -349,6 +363,8 @@ def main():
                     help="solves per mode in the n=150000 time-to-1e-10 comparison (0 = skip)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short run for rocprofv3 counter passes")
+    ap.add_argument("--profile-pass-batch", action="store_true",
+                    help="short batched-SpMM run for rocprofv3 counter passes")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="summarise two counter passes into the traffic file")
     ap.add_argument("--solve-only", action="store_true",
@@ -369,6 +385,9 @@ def main():
         return
     if args.profile_pass:
         profile_pass()
+        return
+    if args.profile_pass_batch:
+        profile_pass_batch()
         return
 
     if args.gpus != 1:

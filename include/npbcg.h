@@ -364,7 +364,8 @@ int npbcg_batch_solve(npbcg_batch *batch, double rtol, int maxit, npbcg_batch_st
                       int *steps, double *relres);
 int npbcg_batch_get_solution(npbcg_batch *batch, double *x);
 /* Autotuner: times the SpMM of this batch for every (layout, lanes_per_row,
- * rows_per_tile) candidate, keeps the fastest, and records it in the text
+ * rows_per_tile) candidate of layouts 1 and 3, keeps the fastest, and
+ * records it in the text
  * file cache_path (may be NULL) under the batch's shape; a later call with the
  * same shape and file reuses the record without timing.  ms (may be NULL)
  * receives the chosen SpMM time. */
