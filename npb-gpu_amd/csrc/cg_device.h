@@ -137,6 +137,44 @@ __device__ __forceinline__ double reduce_partials(const double *__restrict__ par
     return t;
 }
 
+// reduce_partials of two arrays at once: both arrays' loads are issued
+// before either sum is formed, so the two reductions cost one memory round
+// trip.  Each result has the bits reduce_partials gives for its array.
+template <int BLOCK>
+__device__ __forceinline__ void reduce_partials2(const double *__restrict__ pa, int na,
+                                                 const double *__restrict__ pb, int nb,
+                                                 double &ra, double &rb) {
+    static_assert(BLOCK % kBlock == 0, "BLOCK must be a multiple of kBlock");
+    __shared__ double wsum2[2][kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x / kWave) < kWavesPerBlock) {
+        const int t = threadIdx.x;
+        const double a0 = t < na ? pa[t] : 0.0;
+        const double b0 = t < nb ? pb[t] : 0.0;
+        double sa = 0.0, sb = 0.0;
+        sa += a0;
+        sb += b0;
+        for (int i = t + kBlock; i < na; i += kBlock) sa += pa[i];
+        for (int i = t + kBlock; i < nb; i += kBlock) sb += pb[i];
+        sa = segment_sum<64>(sa);
+        sb = segment_sum<64>(sb);
+        if (lane == 0) {
+            wsum2[0][wid] = sa;
+            wsum2[1][wid] = sb;
+        }
+    }
+    __syncthreads();
+    double ta = wsum2[0][0], tb = wsum2[1][0];
+    for (int w = 1; w < kWavesPerBlock; w++) {
+        ta += wsum2[0][w];
+        tb += wsum2[1][w];
+    }
+    __syncthreads();  // wsum2 may be reused by the caller's next reduction
+    ra = ta;
+    rb = tb;
+}
+
 }  // namespace npbcg_dev
 
 #endif

@@ -143,8 +143,11 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     const int c0 = b * W;
     const int wb = min(W, n - c0);
     double beta = 0.0;
-    if constexpr (PUPD)
-        beta = reduce_partials<BLOCK>(rr_part, n_vec) / reduce_partials<BLOCK>(rho_part, n_vec);
+    if constexpr (PUPD) {
+        double rr, rho;
+        reduce_partials2<BLOCK>(rr_part, n_vec, rho_part, n_vec, rr, rho);
+        beta = rr / rho;
+    }
     if constexpr (PUPD) {
         const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
         const double2 *r2 = reinterpret_cast<const double2 *>(r + c0);
@@ -305,8 +308,27 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     const double *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
     const double *__restrict__ minv, const double *__restrict__ tol_part) {
     if (sc->done) return;
-    const double rho = reduce_partials<kBlock>(rho_part, n_rho);
-    const double d = reduce_partials<kBlock>(pq_part, n_pq);
+    // This thread's first row (its block partials of q, p, z, r) is loaded
+    // before the scalar reductions, which it does not depend on, so the two
+    // memory round trips overlap; the sums keep their order.
+    const int j0 = blockIdx.x * kBlock + threadIdx.x;
+    constexpr int kPre = 8;  // block partials of q prefetched
+    double qb[kPre], p0 = 0.0, z0 = 0.0, r0 = 0.0, m0 = 0.0;
+    if (j0 < n) {
+        if (nblk > 0) {
+#pragma unroll
+            for (int b = 0; b < kPre; b++)
+                if (b < nblk) qb[b] = rows_part[(size_t)b * n + j0];
+        } else {
+            qb[0] = q[j0];
+        }
+        p0 = p[j0];
+        z0 = z[j0];
+        r0 = r[j0];
+        if (minv) m0 = minv[j0];
+    }
+    double rho, d;
+    reduce_partials2<kBlock>(rho_part, n_rho, pq_part, n_pq, rho, d);
     const double alpha = rho / d;
     // the stop level: rtol^2 times this start's r0.r0 (tol_part), or times
     // rho itself (plain CG, diagonal PCG)
@@ -322,7 +344,24 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
         sc->steps += 1;
     }
     double acc = 0.0;
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+    if (j0 < n) {
+        double qj;
+        if (nblk > 0) {
+            qj = qb[0];
+#pragma unroll
+            for (int b = 1; b < kPre; b++)
+                if (b < nblk) qj += qb[b];
+            for (int b = kPre; b < nblk; b++) qj += rows_part[(size_t)b * n + j0];
+            q[j0] = qj;
+        } else {
+            qj = qb[0];
+        }
+        z[j0] = z0 + alpha * p0;
+        const double rj = r0 - alpha * qj;
+        r[j0] = rj;
+        acc += minv ? rj * (m0 * rj) : rj * rj;
+    }
+    for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock) {
         double qj;
         if (nblk > 0) {
             qj = rows_part[j];
@@ -347,8 +386,16 @@ __global__ __launch_bounds__(kBlock) void update_p_kernel(
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
     Scalars *sc, const double *__restrict__ minv, const double *__restrict__ stop_part) {
     if (sc->done) return;
-    const double rho = reduce_partials<kBlock>(rr_part, n_vec);
-    const double rho0 = reduce_partials<kBlock>(rho_part, n_vec);
+    // first row loaded ahead of the reductions (as in update_zr_kernel)
+    const int j0 = blockIdx.x * kBlock + threadIdx.x;
+    double r0 = 0.0, p0 = 0.0, m0 = 0.0;
+    if (j0 < n) {
+        r0 = r[j0];
+        p0 = p_in[j0];
+        if (minv) m0 = minv[j0];
+    }
+    double rho, rho0;
+    reduce_partials2<kBlock>(rr_part, n_vec, rho_part, n_vec, rho, rho0);
     const double beta = rho / rho0;
     const double level = stop_part ? reduce_partials<kBlock>(stop_part, n_vec) : rho;
     const bool stop = sc->stop_enabled && fabs(level) <= sc->tol2;
@@ -361,11 +408,12 @@ __global__ __launch_bounds__(kBlock) void update_p_kernel(
         if (stop) sc->done = 1;
     }
     if (stop) return;
+    if (j0 < n) p_out[j0] = minv ? m0 * r0 + beta * p0 : r0 + beta * p0;
     if (minv) {
-        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock)
             p_out[j] = minv[j] * r[j] + beta * p_in[j];
     } else {
-        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock)
             p_out[j] = r[j] + beta * p_in[j];
     }
 }

@@ -43,8 +43,8 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     if (sc->done) return;
     float b32 = 0.0f;
     if constexpr (PUPD) {
-        const double rho = reduce_partials<BLOCK>(rr_part, n_vec);
-        const double rho0 = reduce_partials<BLOCK>(rho_part, n_vec);
+        double rho, rho0;
+        reduce_partials2<BLOCK>(rr_part, n_vec, rho_part, n_vec, rho, rho0);
         const bool stop = fabs(rho) <= sc->tol2;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             sc->rho = rho;
@@ -189,8 +189,23 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
     const float *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
     const float *__restrict__ minv, const double *__restrict__ tol_part) {
     if (sc->done) return;
-    const double rho = reduce_partials<kBlock>(rho_part, gridDim.x);
-    const double dd = reduce_partials<kBlock>(pq_part, n_pq);
+    // This thread's first row (block partials of q, p, d, r) is loaded before
+    // the scalar reductions, which it does not depend on, so the memory round
+    // trips overlap; the sums keep their order.
+    const int j0 = blockIdx.x * kBlock + threadIdx.x;
+    constexpr int kPre = 8;  // block partials of q prefetched
+    float qb[kPre], p0 = 0.0f, d0 = 0.0f, r0 = 0.0f, m0 = 0.0f;
+    if (j0 < n) {
+#pragma unroll
+        for (int b = 0; b < kPre; b++)
+            if (b == 0 || b < nblk) qb[b] = rows_part[(size_t)b * n + j0];
+        p0 = p[j0];
+        d0 = d[j0];
+        r0 = r[j0];
+        if (minv) m0 = minv[j0];
+    }
+    double rho, dd;
+    reduce_partials2<kBlock>(rho_part, gridDim.x, pq_part, n_pq, rho, dd);
     const double alpha = rho / dd;
     // stop level from r0.r0 (tol_part, sparse preconditioner) or rho
     const double rho_tol = (first_step && tol_part) ? reduce_partials<kBlock>(tol_part, gridDim.x) : rho;
@@ -206,7 +221,19 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
     }
     const float a32 = (float)alpha;
     double acc = 0.0;
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+    if (j0 < n) {
+        float qj = qb[0];
+#pragma unroll
+        for (int b = 1; b < kPre; b++)
+            if (b < nblk) qj += qb[b];
+        for (int b = kPre; b < nblk; b++) qj += rows_part[(size_t)b * n + j0];
+        q[j0] = qj;
+        d[j0] = fmaf(a32, p0, d0);
+        const float rj = fmaf(-a32, qj, r0);
+        r[j0] = rj;
+        acc += (double)rj * (double)(minv ? m0 * rj : rj);
+    }
+    for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock) {
         float qj = rows_part[j];
         for (int b = 1; b < nblk; b++) qj += rows_part[(size_t)b * n + j];
         q[j] = qj;
@@ -225,8 +252,16 @@ __global__ __launch_bounds__(kBlock) void update_p32_kernel(
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, Scalars *sc,
     const float *__restrict__ minv, const double *__restrict__ stop_part) {
     if (sc->done) return;
-    const double rho = reduce_partials<kBlock>(rr_part, gridDim.x);
-    const double rho0 = reduce_partials<kBlock>(rho_part, gridDim.x);
+    // first row loaded ahead of the reductions (as in update_zr32_kernel)
+    const int j0 = blockIdx.x * kBlock + threadIdx.x;
+    float r0 = 0.0f, p0 = 0.0f, m0 = 0.0f;
+    if (j0 < n) {
+        r0 = r[j0];
+        p0 = p[j0];
+        if (minv) m0 = minv[j0];
+    }
+    double rho, rho0;
+    reduce_partials2<kBlock>(rr_part, gridDim.x, rho_part, gridDim.x, rho, rho0);
     const double beta = rho / rho0;
     const double level = stop_part ? reduce_partials<kBlock>(stop_part, gridDim.x) : rho;
     const bool stop = fabs(level) <= sc->tol2;
@@ -237,11 +272,12 @@ __global__ __launch_bounds__(kBlock) void update_p32_kernel(
     }
     if (stop) return;
     const float b32 = (float)beta;
+    if (j0 < n) p[j0] = minv ? fmaf(b32, p0, m0 * r0) : fmaf(b32, p0, r0);
     if (minv) {
-        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock)
             p[j] = fmaf(b32, p[j], minv[j] * r[j]);
     } else {
-        for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock)
             p[j] = fmaf(b32, p[j], r[j]);
     }
 }
