@@ -74,7 +74,9 @@ int orc_makea(int n, int nonzer, double rcond, double shift, double seed,
               long nz_cap, int *rowstr, int *colidx, double *a,
               double *ratio_out)
 {
-    if (n < 1 || nonzer < 0 || nonzer >= n || nz_cap < 1 || !rowstr ||
+    /* nonzer <= n: sprnvc (cg.cu:1798-1833) draws nonzer distinct positions
+       in 1..n, which terminates for any nonzer up to n. */
+    if (n < 1 || nonzer < 0 || nonzer > n || nz_cap < 1 || !rowstr ||
         !colidx || !a)
         return -1;
     const int w = nonzer + 1; /* entries per outer-product vector */

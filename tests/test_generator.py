@@ -24,8 +24,11 @@ def _same(m, o):
 
 @pytest.mark.parametrize("n,nonzer,shift,seed", [
     (1, 0, 10.0, 3.0),          # one row: only the forced diagonal
+    (1, 1, 10.0, 7.0),          # nonzer = n: the one position drawn
     (2, 1, 1.0, 5.0),
-    (17, 16, 2.0, 11.0),        # nonzer = n - 1: every position drawn
+    (3, 3, 10.0, 1003.0),       # nonzer = n: every position drawn
+    (17, 16, 2.0, 11.0),        # nonzer = n - 1
+    (65, 5, 10.0, 1065.0),
     (100, 0, 1.5, 12.0),        # diagonal-only matrix
     (1000, 3, 4.0, 271828.0),   # even seed
     (1400, 7, 10.0, 12345.0),
