@@ -160,6 +160,20 @@ int npbcg_describe(const npbcg_ctx *ctx, char *buf, int len);
  * from HIP events.  Used to measure SpMV bandwidth in isolation. */
 int npbcg_spmv_bench(npbcg_ctx *ctx, int reps, double *avg_ms);
 
+/* Single-system autotuner, in place of the reference's compile-time launch
+ * configuration (config/gpu.config:32-42, setparams.cpp:256-332): times the
+ * q = A p SpMV of this matrix through temporary contexts (created with *opts,
+ * other fields unchanged) for the LDS-staged layout at its default shape,
+ * with twice the row ranges and with half-width column blocks, and for the
+ * four CSR row kernels; writes the fastest into opts->spmv_variant,
+ * tile_width and tile_rows, for the caller's npbcg_create.  The text file
+ * cache_path (may be NULL) keeps one line per tuning, "<key> <variant>
+ * <width> <rows> <ms>", keyed by n, nnz, a digest of the row lengths and the
+ * CU count; a later call with the same key reads it instead of timing.  ms
+ * (may be NULL) receives the chosen SpMV time. */
+int npbcg_tune(int n, const int *rowstr, const int *colidx, const double *a, npbcg_opts *opts,
+               const char *cache_path, double *ms);
+
 /* A plain device-to-device copy of `bytes` (16-byte vector loads and
  * stores, grid-stride) repeated reps times on the context's device;
  * *avg_ms is the mean duration.  It has no counterpart in the reference: it

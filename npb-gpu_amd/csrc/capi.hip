@@ -831,6 +831,144 @@ int npbcg_spmv_bench(npbcg_ctx *c, int reps, double *avg_ms) {
     return NPBCG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Record key of a single-system tuning: the order, the nonzeros, an FNV-1a
+// digest of the row lengths and the CU count.
+std::string tune_key(int n, const int *rowstr, int cus) {
+    uint64_t h = 1469598103934665603ull;
+    for (int j = 0; j < n; j++) {
+        uint32_t len = (uint32_t)(rowstr[j + 1] - rowstr[j]);
+        for (int k = 0; k < 4; k++) {
+            h ^= (len >> (8 * k)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+    }
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "npbcg1:n=%d:nnz=%d:rows=%016llx:cus=%d", n, rowstr[n],
+                  (unsigned long long)h, cus);
+    return buf;
+}
+
+// "<key> <variant> <width> <rows> <ms>" per line; the last record of a key wins.
+bool tune_lookup(const char *path, const std::string &key, int *variant, int *w, int *r,
+                 double *ms) {
+    if (!path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    char k[512];
+    int v = 0, a = 0, b = 0;
+    double t = 0.0;
+    bool found = false;
+    while (std::fscanf(f, "%511s %d %d %d %lf", k, &v, &a, &b, &t) == 5)
+        if (key == k) {
+            *variant = v;
+            *w = a;
+            *r = b;
+            *ms = t;
+            found = true;
+        }
+    std::fclose(f);
+    return found;
+}
+
+void tune_store(const char *path, const std::string &key, int variant, int w, int r, double ms) {
+    if (!path) return;
+    FILE *f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), variant, w, r, ms);
+    std::fclose(f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int npbcg_tune(int n, const int *rowstr, const int *colidx, const double *a, npbcg_opts *opts,
+               const char *cache_path, double *ms_out) {
+    if (n < 1 || !rowstr || !colidx || !a || !opts)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_tune: invalid argument");
+    const npbcg_opts base = *opts;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+        return set_error(NPBCG_ERR_NODEVICE, "npbcg_tune: no GPU");
+    const int device = (base.device >= 0 && base.device < count) ? base.device : 0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        cus = 0;
+    const std::string key = tune_key(n, rowstr, cus);
+    int best_v = 0, best_w = 0, best_r = 0;
+    double best = 0.0;
+    if (!tune_lookup(cache_path, key, &best_v, &best_w, &best_r, &best)) {
+        best = std::numeric_limits<double>::infinity();
+        // Times one shape through a temporary context; a request the context
+        // does not honour (e.g. the LDS-staged layout on unsorted rows) is
+        // skipped.  W and R receive the LDS-staged shape actually built.
+        auto time_one = [&](int variant, int w, int r, int *W, int *R) -> int {
+            npbcg_opts o = base;
+            o.spmv_variant = variant;
+            o.tile_width = w;
+            o.tile_rows = r;
+            npbcg_ctx *c = nullptr;
+            int rc = npbcg_create(&c, n, rowstr, colidx, a, &o);
+            if (rc != NPBCG_OK) return rc;
+            double t = std::numeric_limits<double>::infinity();
+            if (c->pb.spmv_variant == variant) {
+                for (int trial = 0; trial < 2 && rc == NPBCG_OK; trial++) {
+                    double ms = 0.0;
+                    rc = npbcg_spmv_bench(c, 20, &ms);
+                    t = std::min(t, ms);
+                }
+            }
+            const int bw = variant == npbcg_dev::kSpmvTiled ? c->pb.sell.W : 0;
+            const int br = variant == npbcg_dev::kSpmvTiled ? c->pb.sell.R : 0;
+            npbcg_destroy(c);
+            if (rc != NPBCG_OK) return rc;
+            if (W) *W = bw;
+            if (R) *R = br;
+            if (t < best) {
+                best = t;
+                best_v = variant;
+                best_w = bw;
+                best_r = br;
+            }
+            return NPBCG_OK;
+        };
+        // the LDS-staged layout at its default shape, with twice the row
+        // ranges, and with half-width column blocks (two workgroups per CU)
+        int W0 = 0, R0 = 0;
+        int rc = time_one(npbcg_dev::kSpmvTiled, 0, 0, &W0, &R0);
+        if (rc != NPBCG_OK) return rc;
+        if (W0 > 0) {
+            if (R0 > 1) {
+                rc = time_one(npbcg_dev::kSpmvTiled, W0, (R0 + 1) / 2, nullptr, nullptr);
+                if (rc != NPBCG_OK) return rc;
+            }
+            const int nblk = (n + W0 - 1) / W0;
+            const int w2 = (((n + 2 * nblk - 1) / (2 * nblk)) + 1) & ~1;
+            if (w2 >= 2 && w2 < W0) {
+                rc = time_one(npbcg_dev::kSpmvTiled, w2, 0, nullptr, nullptr);
+                if (rc != NPBCG_OK) return rc;
+            }
+        }
+        for (int v : {npbcg_dev::kSpmvSeg64, npbcg_dev::kSpmvSeg32, npbcg_dev::kSpmvSeg16,
+                      npbcg_dev::kSpmvSeg64U2}) {
+            rc = time_one(v, 0, 0, nullptr, nullptr);
+            if (rc != NPBCG_OK) return rc;
+        }
+        tune_store(cache_path, key, best_v, best_w, best_r, best);
+    }
+    opts->spmv_variant = best_v;
+    opts->tile_width = best_w;
+    opts->tile_rows = best_r;
+    if (ms_out) *ms_out = best;
+    clear_error();
+    return NPBCG_OK;
+}
+
+
 int npbcg_copy_bench(npbcg_ctx *c, long long bytes, int reps, double *avg_ms) {
     if (!c || bytes < 16 || reps < 1 || !avg_ms)
         return set_error(NPBCG_ERR_INVALID, "npbcg_copy_bench: invalid argument");

@@ -147,6 +147,8 @@ def _load() -> ctypes.CDLL:
         "npbcg_set_spmv_sampling": (ctypes.c_int, [vp, ctypes.c_int]),
         "npbcg_get_timing": (ctypes.c_int, [vp, ctypes.POINTER(Timing)]),
         "npbcg_spmv_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
+        "npbcg_tune": (ctypes.c_int, [ctypes.c_int, _I32P, _I32P, _F64P, ctypes.c_void_p,
+                                      ctypes.c_char_p, _DP]),
         "npbcg_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
         "npbcg_copy_bench": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int, _DP]),
         "npbcg_set_rhs": (ctypes.c_int, [vp, _F64P]),
@@ -256,6 +258,18 @@ def makea(n: int, nonzer: int, shift: float, seed: float = 314159265.0,
                                  colidx, a, ctypes.byref(ratio)), "npbcg_makea")
     return Matrix(n, rowstr, colidx[:nnz].copy(), a[:nnz].copy(), ratio.value, nonzer,
                   shift, rcond, seed)
+
+
+def tune(m: Matrix, cache_path: Optional[str] = None, **opts):
+    """Single-system autotuner (npbcg_tune): returns ({spmv_variant,
+    tile_width, tile_rows}, SpMV ms) for Context(m, **opts, **shape)."""
+    o = default_opts(**opts)
+    ms = ctypes.c_double(0.0)
+    _check(LIB.npbcg_tune(m.n, m.rowstr, m.colidx, m.a, ctypes.byref(o),
+                          None if cache_path is None else cache_path.encode(), ctypes.byref(ms)),
+           "npbcg_tune")
+    return ({"spmv_variant": o.spmv_variant, "tile_width": o.tile_width,
+             "tile_rows": o.tile_rows}, ms.value)
 
 
 class Context:

@@ -223,3 +223,28 @@ def test_vector_roundtrip_and_sampling(gpu, golden):
         t = ctx.timing()
         assert t.spmv_samples == 25 and t.spmv_avg_ms > 0 and t.total_ms > 0
     _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"][:2]], "sampled run")
+
+
+@pytest.mark.parametrize("size", ["cfg1", "n20000"])
+def test_single_system_tuner_persists_and_matches_oracle(gpu, golden, tmp_path, size):
+    if size == "cfg1":
+        c = golden["cfg1_n1400_seed12345"]
+        m, niter, shift = _gen(gpu, c), c["niter"], c["shift"]
+    else:
+        m, niter, shift = gpu.makea(20000, 11, 30.0, 271.0), 3, 30.0
+    cache = str(tmp_path / "tune.txt")
+    shape, ms = gpu.tune(m, cache)
+    print(f"{size}: tuned {shape}, {ms * 1e3:.2f} us")
+    assert shape["spmv_variant"] in (1, 2, 3, 4, 5) and ms > 0
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1
+    assert lines[0].split()[1:4] == [str(shape[k]) for k in ("spmv_variant", "tile_width",
+                                                              "tile_rows")]
+    shape2, ms2 = gpu.tune(m, cache)  # the record is read back, not re-timed
+    assert shape2 == shape and abs(ms2 - ms) <= 1e-5
+    assert len(open(cache).read().splitlines()) == 1
+    with gpu.Context(m, **shape) as ctx:
+        assert ctx.describe()["spmv_variant"] == shape["spmv_variant"]
+        zeta, rnorm = ctx.power(niter, shift)
+    ozeta, _, _ = oracle.power(_ocsr(m), niter)
+    _zeta_close(zeta, ozeta, f"{size} tuned shape vs oracle")
