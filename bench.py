@@ -76,30 +76,52 @@ def golden_zeta():
 
 
 def cpu_baseline(m, wl, outer_iters: int):
-    """Serial oracle power loop (single thread, pinned) on `outer_iters`
-    outer iterations of the same matrix."""
+    """The reference's serial power loop (conj_grad + zeta + x = z/||z||,
+    cg.cu:458-482 / HIP/CG/cg.cpp) on `outer_iters` outer iterations of the
+    same matrix, one pinned thread.  It runs the reference build under
+    oracle/_ref (compiled from the reference sources by oracle/Makefile) when
+    that build is present and its class matches the workload; otherwise the
+    oracle's C restatement of the same loop."""
     import oracle
 
-    om = oracle.CSR(m.n, m.rowstr, m.colidx, m.a, m.ratio, wl["nonzer"], wl["shift"],
-                    wl["rcond"], wl["seed"])
+    ref = None
+    if oracle.ref_available("B"):
+        ref = oracle.RefClass("B")
+        if (ref.n, ref.nonzer, ref.shift) != (wl["n"], wl["nonzer"], wl["shift"]):
+            ref = None
+    same_matrix = None
+    if ref is not None:
+        rm = ref.makea(wl["seed"])  # the reference's own generator, outside the timing
+        same_matrix = bool(np.array_equal(rm.rowstr, m.rowstr)
+                           and np.array_equal(rm.colidx, m.colidx)
+                           and np.array_equal(rm.a.view(np.int64), m.a.view(np.int64)))
+    else:
+        om = oracle.CSR(m.n, m.rowstr, m.colidx, m.a, m.ratio, wl["nonzer"], wl["shift"],
+                        wl["rcond"], wl["seed"])
     old = os.sched_getaffinity(0)
     core = min(old)
     os.sched_setaffinity(0, {core})
     try:
         t0 = time.perf_counter()
-        zeta, _, _ = oracle.power(om, outer_iters, wl["cgitmax"])
+        if ref is not None:
+            zeta, _, _ = ref.power(outer_iters)
+        else:
+            zeta, _, _ = oracle.power(om, outer_iters, wl["cgitmax"])
         dt = time.perf_counter() - t0
     finally:
         os.sched_setaffinity(0, old)
     gbytes = outer_iters * outer_iter_bytes(m.n, m.nnz, wl["cgitmax"]) / 1e9
-    return {
+    what = ("reference HIP/CG/cg.cpp serial power loop (oracle/_ref/libnpbref_B.so, built from "
+            "the reference sources)" if ref is not None else
+            "oracle/npbcg_oracle.c serial power loop")
+    out = {
         "value": round(gbytes / dt, 3),
         "unit": "GB/s",
         "cores": 1,
-        "kind": "port",
+        "kind": "reference" if ref is not None else "port",
         "sample": (f"{outer_iters} of {wl['niter']} outer iterations (x {wl['cgitmax']} CG "
-                   f"steps) of the same n={m.n} matrix, oracle/npbcg_oracle.c serial power "
-                   f"loop, one pinned thread, {dt:.2f} s"),
+                   f"steps) of the same n={m.n} matrix, {what}, one pinned thread, "
+                   f"{dt:.2f} s"),
         "seconds": round(dt, 3),
         "outer_iters_per_s": round(outer_iters / dt, 4),
         "cg_iters_per_s": round(outer_iters * wl["cgitmax"] / dt, 2),
@@ -107,6 +129,9 @@ def cpu_baseline(m, wl, outer_iters: int):
         "host_nproc": os.cpu_count(),
         "zeta_last": float(zeta[-1]),
     }
+    if same_matrix is not None:
+        out["reference_matrix_bit_identical"] = same_matrix
+    return out
 
 
 def _cpu_model() -> str:
