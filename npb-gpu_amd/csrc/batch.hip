@@ -131,14 +131,15 @@ constexpr int kXcds = 8;  // MI355X: workgroup b of a launch runs on XCD b % 8
 // system s owns units [sys_first[s], sys_first[s+1]).  With at least kXcds
 // systems, whole systems are placed on XCDs (greedy by nonzeros, largest
 // first) so that a system's vector is read from one XCD's L2; the lists are
-// interleaved to match the round-robin dispatch and padded with -1.  Fewer
-// systems keep the plain order.
+// interleaved to match the round-robin dispatch.  Fewer systems keep the
+// plain order.  Either way the length is a multiple of kXcds, padded with -1.
 std::vector<int> xcd_order(const npbcg_batch *b, const std::vector<int> &sys_tile,
                            const std::vector<long long> &sys_nnz) {
     const int tiles = sys_tile[b->nsys];
     std::vector<int> out;
-    if (b->nsys < kXcds) {
-        out.resize(tiles);
+    if (b->nsys < kXcds) {  // padded to a multiple of kXcds (the SELL SpMM's
+                            // reversed order maps slots within groups of 8)
+        out.assign(((size_t)tiles + kXcds - 1) / kXcds * kXcds, -1);
         for (int t = 0; t < tiles; t++) out[t] = t;
         return out;
     }
@@ -447,9 +448,10 @@ void cache_store(const char *path, const std::string &key, int lpr, int rt, int 
 template <typename T>
 int time_spmm(npbcg_batch *b, int reps, double *ms) {
     const BatchVecs<T> v = inner_vecs<T>(b);
-    B_HIP(batch_spmm<T>(b->bd, v, b->stream));  // warm
+    B_HIP(batch_spmm<T>(b->bd, v, 1, b->stream));  // warm
     B_HIP(hipEventRecord(b->ev0, b->stream));
-    for (int i = 0; i < reps; i++) B_HIP(batch_spmm<T>(b->bd, v, b->stream));
+    // alternating item order, as in the solve's CG steps
+    for (int i = 0; i < reps; i++) B_HIP(batch_spmm<T>(b->bd, v, i & 1, b->stream));
     B_HIP(hipEventRecord(b->ev1, b->stream));
     B_HIP(hipEventSynchronize(b->ev1));
     float t = 0;

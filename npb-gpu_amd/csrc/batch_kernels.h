@@ -94,9 +94,10 @@ hipError_t batch_prepare_resid(int bytes);
 // Workgroup size of the SELL SpMM: one 1024-thread workgroup per CU, which
 // holds a column block of up to 128 KB in LDS.
 constexpr int kSellBlock = 1024;
-// q = A p and p.q partials only (the SpMM the autotuner times).
+// q = A p and p.q partials only (the SpMM the autotuner times); rev: item
+// order of the SELL SpMM (0 forward, 1 backward within each XCD).
 template <typename T>
-hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st);
+hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, int rev, hipStream_t st);
 // r = p = (T) res, d = 0 for the unfinished right-hand sides.
 template <typename T>
 hipError_t batch_init(const BatchDev &bd, const BatchVecs<T> &v, const double *res,

@@ -286,16 +286,24 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_cb_kernel(BatchDev bd,
 // next chunk (or of the next slice's first chunk and its lane rows) are issued
 // before the current chunk's LDS gathers and FMAs, and the first chunk is
 // issued before the LDS fill.
+//
+// rev = 1 takes each XCD's items in reverse order (workgroup 8 i + x runs the
+// item listed at 8 (L - 1 - i) + x, L = item_blocks / 8): successive SpMMs
+// alternate, so each starts on the items the last one streamed last, which
+// the memory-side cache still holds.  Systems keep their XCD.
 template <typename T, int NR, int BLOCK, int UNR, bool RESID>
 __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
                                                                 const T *__restrict__ sval,
                                                                 const T *__restrict__ p,
                                                                 T *__restrict__ rows_part,
-                                                                double *__restrict__ pq_part) {
+                                                                double *__restrict__ pq_part,
+                                                                int rev) {
     extern __shared__ double2 lds_sell[];  // 16-byte aligned
     T *vlds = reinterpret_cast<T *>(lds_sell);
     constexpr int kWaves = BLOCK / kWave;
-    const int it = bd.block_item[blockIdx.x];
+    const int slot = rev ? bd.item_blocks - 8 - 8 * (int)(blockIdx.x / 8) + (int)(blockIdx.x % 8)
+                         : (int)blockIdx.x;
+    const int it = bd.block_item[slot];
     if (it < 0) return;
     const int s = bd.item[4 * it];
     if (!RESID && system_done<NR>(bd.done, s)) return;
@@ -708,13 +716,14 @@ constexpr int sell_unr() {
     return sizeof(T) == 8 && NR == 4 ? 4 : 8;
 }
 
-// q = A p (row sums, or block row partials for cb) and p.q partials.
+// q = A p (row sums, or block row partials for cb) and p.q partials; rev
+// selects the SELL SpMM's item order (alternated by the callers).
 template <typename T, int NR, int LPR>
-void launch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st) {
+void launch_spmm(const BatchDev &bd, const BatchVecs<T> &v, int rev, hipStream_t st) {
     if (bd.sell) {
         batch_spmm_sell_kernel<T, NR, kSellBlock, sell_unr<T, NR>(), false>
             <<<bd.item_blocks, kSellBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
-                bd, v.sval, v.p, v.rows_part, v.part_pq);
+                bd, v.sval, v.p, v.rows_part, v.part_pq, rev);
     } else if (bd.cb) {
         batch_spmm_cb_kernel<T, NR, LPR><<<bd.items, kBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
             bd, v.val, v.p, v.rows_part, v.part_pq);
@@ -735,7 +744,7 @@ hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, in
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            launch_spmm<T, NR, LPR>(bd, v, st);
+            launch_spmm<T, NR, LPR>(bd, v, step & 1, st);
             return hipGetLastError();
         });
     });
@@ -756,12 +765,12 @@ hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, in
 }
 
 template <typename T>
-hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, hipStream_t st) {
+hipError_t batch_spmm(const BatchDev &bd, const BatchVecs<T> &v, int rev, hipStream_t st) {
     return with_nr(bd.nr, [&](auto nrc) {
         constexpr int NR = decltype(nrc)::value;
         return with_lpr(bd.lpr, [&](auto lc) {
             constexpr int LPR = decltype(lc)::value;
-            launch_spmm<T, NR, LPR>(bd, v, st);
+            launch_spmm<T, NR, LPR>(bd, v, rev, st);
             return hipGetLastError();
         });
     });
@@ -847,7 +856,7 @@ hipError_t batch_residual_sell(const BatchDev &bd, const double *sval, const dou
         constexpr int NR = decltype(nrc)::value;
         batch_spmm_sell_kernel<double, NR, kSellBlock, sell_unr<double, NR>(), true>
             <<<bd.item_blocks, kSellBlock, (size_t)bd.cbw * NR * sizeof(double), st>>>(
-                bd, sval, x, rows_part, nullptr);
+                bd, sval, x, rows_part, nullptr, 0);
         hipError_t e2 = hipGetLastError();
         if (e2 != hipSuccess) return e2;
         batch_resid_combine_kernel<NR><<<bd.tiles, kBlock, 0, st>>>(bd, rows_part, b, res, part);
@@ -882,8 +891,9 @@ template hipError_t batch_cg_step<double>(const BatchDev &, const BatchVecs<doub
                                           hipStream_t);
 template hipError_t batch_prepare_cb<float>(int);
 template hipError_t batch_prepare_cb<double>(int);
-template hipError_t batch_spmm<float>(const BatchDev &, const BatchVecs<float> &, hipStream_t);
-template hipError_t batch_spmm<double>(const BatchDev &, const BatchVecs<double> &, hipStream_t);
+template hipError_t batch_spmm<float>(const BatchDev &, const BatchVecs<float> &, int, hipStream_t);
+template hipError_t batch_spmm<double>(const BatchDev &, const BatchVecs<double> &, int,
+                                       hipStream_t);
 template hipError_t batch_init<float>(const BatchDev &, const BatchVecs<float> &, const double *,
                                       hipStream_t);
 template hipError_t batch_init<double>(const BatchDev &, const BatchVecs<double> &,

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     double *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W,
     int nrange, const double *__restrict__ r, double *__restrict__ p_new,
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
-    const Scalars *__restrict__ sc) {
+    const Scalars *__restrict__ sc, int rev) {
     if (MODE == 0 && sc->done) return;  // a solve already met its tolerance
     if (MODE == 1 && sc->skip_resid) return;  // a refinement sweep that is not needed
     // Dynamic LDS: W vector entries, then one p.q slot per slice of this
@@ -183,6 +183,13 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
         const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
         return k < nsl ? k : -1;
     };
+    // This wave's rounds, walked forward or (rev) backward: successive SpMVs
+    // alternate, so each pass starts on the slices the previous one streamed
+    // last, which the memory-side cache still holds when the matrix is about
+    // its size.  Only the order of the slices changes, not any sum.
+    int nrw = 0;
+    while (slice_of(nrw) >= 0) nrw++;
+    auto slice_at = [&](int i) { return i < nrw ? slice_of(rev ? nrw - 1 - i : i) : -1; };
     // The entry stream is software-pipelined in chunks of UNR entries
     // (sell_chunk): the next chunk's loads, or the next slice's first chunk
     // and lane rows, go out before the current chunk's LDS gathers.  The first
@@ -190,7 +197,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     // staged, so its memory latency overlaps the LDS fill.
     double na[UNR];
     int nc[UNR];
-    int cur = slice_of(0), cbase = 0, cwidth = 0, nrow = -1;
+    int cur = slice_at(0), cbase = 0, cwidth = 0, nrow = -1;
     if (cur >= 0) {
         cbase = slice_off[s0 + cur];
         cwidth = (slice_off[s0 + cur + 1] - cbase) / kWave;
@@ -205,9 +212,9 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     }
     __syncthreads();
     double *part_b = rows_part + (size_t)b * n;
-    for (int rd = 0; cur >= 0; rd++) {
+    for (int i = 0; cur >= 0; i++) {
         const int k = cur, width = cwidth, base = cbase, row = nrow;
-        const int nxt = slice_of(rd + 1);
+        const int nxt = slice_at(i + 1);
         int xbase = 0, xwidth = 0;
         if (nxt >= 0) {
             xbase = slice_off[s0 + nxt];
@@ -596,25 +603,28 @@ const double *rho_source(const Fp64Problem &pb, int step) {
 }
 
 // MODE 0 at `step` (q = A p): for step > 0 the kernel forms p of this step
-// from r and the previous step's p.  MODE 1: r = A z.
+// from r and the previous step's p.  MODE 1: r = A z.  Slice order: forward
+// on even steps, backward on odd ones and for r = A z, which follows the odd
+// count of CG steps of an outer iteration.
 template <int BLOCK, int MODE>
 hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEvent_t e0,
                          hipEvent_t e1) {
     const SellDev &d = pb.sell;
+    const int rev = MODE == 0 ? (step & 1) : 1;
     if (MODE == 0 && step > 0 && fused_p(pb)) {
         return launch_k(spmv_sell_kernel<BLOCK, 8, 0, true>, d.grid, BLOCK, d.lds_bytes, st,
                         e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col,
                         (const double *)p_of_step(pb, step - 1), d.rows_part, pb.part.pq,
                         pb.n, d.W, d.nrange, (const double *)pb.r, p_of_step(pb, step),
                         (const double *)pb.part.rr[(step - 1) & 1],
-                        rho_source(pb, step - 1), pb.vec_blocks, (const Scalars *)pb.sc);
+                        rho_source(pb, step - 1), pb.vec_blocks, (const Scalars *)pb.sc, rev);
     }
     const double *v = MODE == 0 ? p_of_step(pb, step) : pb.z;
     return launch_k(spmv_sell_kernel<BLOCK, 8, MODE, false>, d.grid, BLOCK, d.lds_bytes, st,
                     e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v, d.rows_part,
                     pb.part.pq, pb.n, d.W, d.nrange, (const double *)nullptr,
                     (double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0,
-                    (const Scalars *)pb.sc);
+                    (const Scalars *)pb.sc, rev);
 }
 
 template <int MODE>

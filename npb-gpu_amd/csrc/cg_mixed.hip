@@ -39,7 +39,8 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     const uint16_t *__restrict__ col, const float *__restrict__ v,
     float *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W, int nrange,
     Scalars *sc, const float *__restrict__ r, float *__restrict__ p_new,
-    const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec) {
+    const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
+    int rev) {
     if (sc->done) return;
     float b32 = 0.0f;
     if constexpr (PUPD) {
@@ -72,12 +73,18 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
         const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
         return k < nsl ? k : -1;
     };
+    // This wave's rounds forward or (rev) backward: successive SpMVs
+    // alternate so each starts on the slices the last one streamed last (held
+    // by the memory-side cache); no sum changes.
+    int nrw = 0;
+    while (slice_of(nrw) >= 0) nrw++;
+    auto slice_at = [&](int i) { return i < nrw ? slice_of(rev ? nrw - 1 - i : i) : -1; };
     // Software-pipelined entry stream (sell_chunk): the next chunk's loads, or
     // the next slice's first chunk and lane rows, go out before the current
     // chunk's LDS gathers; the first chunk is loaded before the LDS fill.
     float na[UNR];
     int nc[UNR];
-    int cur = slice_of(0), cbase = 0, cwidth = 0, nrow = -1;
+    int cur = slice_at(0), cbase = 0, cwidth = 0, nrow = -1;
     if (cur >= 0) {
         cbase = slice_off[s0 + cur];
         cwidth = (slice_off[s0 + cur + 1] - cbase) / kWave;
@@ -114,9 +121,9 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     }
     __syncthreads();
     float *part_b = rows_part + (size_t)b * n;
-    for (int rd = 0; cur >= 0; rd++) {
+    for (int i = 0; cur >= 0; i++) {
         const int k = cur, width = cwidth, base = cbase, row = nrow;
-        const int nxt = slice_of(rd + 1);
+        const int nxt = slice_at(i + 1);
         int xbase = 0, xwidth = 0;
         if (nxt >= 0) {
             xbase = slice_off[s0 + nxt];
@@ -368,16 +375,16 @@ hipError_t prepare_mixed_kernels(int lds_bytes) {
 template <bool PUPD>
 static hipError_t launch_spmv32(const Fp64Problem &pb, const MixedProblem &mp, const float *v,
                                 float *p_new, const double *rr_part, const double *rho_part,
-                                hipStream_t st) {
+                                int rev, hipStream_t st) {
     const SellDev &sd = pb.sell;
     if (sd.block == 512)
         spmv_sell32_kernel<512, 8, PUPD><<<sd.grid, 512, mp.lds_bytes, st>>>(
             sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
-            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks);
+            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev);
     else
         spmv_sell32_kernel<1024, 8, PUPD><<<sd.grid, 1024, mp.lds_bytes, st>>>(
             sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
-            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks);
+            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev);
     return hipGetLastError();
 }
 
@@ -407,9 +414,9 @@ hipError_t enqueue_inner32(const Fp64Problem &pb, const MixedProblem &mp, int fi
         float *p_it = fused ? pbuf[it & 1] : mp.p;
         if (fused && it > 0)
             e = launch_spmv32<true>(pb, mp, pbuf[(it - 1) & 1], p_it, mp.part_rr[(it - 1) & 1],
-                                    rho32_source(mp, it - 1), st);
+                                    rho32_source(mp, it - 1), it & 1, st);
         else
-            e = launch_spmv32<false>(pb, mp, p_it, nullptr, nullptr, nullptr, st);
+            e = launch_spmv32<false>(pb, mp, p_it, nullptr, nullptr, nullptr, it & 1, st);
         if (e != hipSuccess) break;
         // sparse preconditioner: alpha from the previous step's r.z (the
         // start's in rz[1]); beta from r.z; the stop test on r.r
