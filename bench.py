@@ -384,6 +384,8 @@ def main():
     ap.add_argument("--spmv-variant", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fuse-p-update", action="store_true")
+    ap.add_argument("--cg-form", type=int, default=0,
+                    help="0 auto (one-reduction form), 1 the reference's step order, 2 one-reduction")
     ap.add_argument("--solve-reps", type=int, default=10,
                     help="solves per mode in the n=150000 time-to-1e-10 comparison (0 = skip)")
     ap.add_argument("--profile-pass", action="store_true",
@@ -429,7 +431,7 @@ def main():
     t_gen = time.perf_counter() - t0
     ctx = npbcg.Context(m, device=0, cgitmax=wl["cgitmax"], spmv_variant=args.spmv_variant,
                         use_graph=0 if args.no_graph else 1,
-                        fuse_p_update=1 if args.fuse_p_update else 0)
+                        fuse_p_update=1 if args.fuse_p_update else 0, cg_form=args.cg_form)
     n, nnz = m.n, m.nnz
 
     for _ in range(args.warmup):
@@ -466,7 +468,8 @@ def main():
     traffic = load_traffic()
     roofline = {
         "bound": "hbm",
-        "kernel": "spmv_pq_kernel (q = A p, p.q)",
+        "kernel": ("spmv_sell_kernel, the CG SpMV: s = A r with r.s partials (one-reduction "
+                   "form; q = A p with p.q in the reference's step order, --cg-form 1)"),
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

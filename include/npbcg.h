@@ -56,7 +56,13 @@ typedef struct npbcg_opts {
                          instead of a separate launch (default 0) */
     int solve_batch;  /* npbcg_solve: CG steps launched before the first host check
                          (later checks use half); 0 -> 10 */
-    int reserved[7];
+    int cg_form;      /* conj_grad / npbcg_power on the LDS-staged layout: 0 = auto (the
+                         one-reduction form), 1 = the reference's step order (q = A p;
+                         alpha; z, r; rho; beta; p, three launches a step), 2 = the
+                         one-reduction form: s = A r, then one update forming p, q = A p,
+                         p.q = r.s - beta rho / alpha_prev, z and r (two launches a step).
+                         Solves always use the reference's step order. */
+    int reserved[6];
 } npbcg_opts;
 
 /* Timing of the last npbcg_power / npbcg_conj_grad_device call, from HIP

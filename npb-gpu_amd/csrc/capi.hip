@@ -207,6 +207,7 @@ void refresh_solve_view(npbcg_ctx *c) {
     c->pbs.x = vec_ptr(c, kVecB);
     c->pbs.z = vec_ptr(c, kVecXsol);
     c->pbs.fuse_p_update = 0;  // a solve may stop after any step; p stays in place
+    c->pbs.cg_form = 0;        // the stop test sits in the reference-order p update
 }
 
 void drop_graph(npbcg_ctx *c) {
@@ -538,6 +539,10 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     c->pb.spmv_blocks = spmv_blocks;
     c->pb.spmv_variant = variant;
     c->pb.fuse_p_update = c->opts.fuse_p_update ? 1 : 0;
+    // one-reduction CG form: LDS-staged layout (s = A r goes to its block row
+    // partials) and no fused p update
+    c->pb.cg_form = (variant == npbcg_dev::kSpmvTiled && !c->pb.fuse_p_update &&
+                     c->opts.cg_form != 1) ? 1 : 0;
     c->pb.sell = sd;
     refresh_solve_view(c);
     CREATE_HIP(npbcg_dev::launch_fill(c->pb.x, n + 1, 1.0, vec_blocks, c->stream));
@@ -803,11 +808,12 @@ int npbcg_describe(const npbcg_ctx *c, char *buf, int len) {
         "\"csr_blocks\": %d, \"block_width\": %d, \"col_blocks\": %d, "
         "\"range_rows\": %d, \"row_ranges\": %d, \"sell_workgroups\": %d, "
         "\"sell_threads\": %d, \"sell_stored\": %lld, \"cgitmax\": %d, "
-        "\"use_graph\": %d, \"fuse_p_update\": %d, \"device_name\": \"%s\", "
-        "\"compute_units\": %d}",
+        "\"use_graph\": %d, \"fuse_p_update\": %d, \"cg_form\": %d, "
+        "\"device_name\": \"%s\", \"compute_units\": %d}",
         c->n, c->nnz, c->pb.spmv_variant, c->pb.vec_blocks, c->pb.spmv_blocks, sd.W,
         sd.nblk, sd.R, sd.nrange, sd.grid, sd.block, c->sell_stored, c->cgitmax,
-        c->opts.use_graph, c->pb.fuse_p_update, prop.name, prop.multiProcessorCount);
+        c->opts.use_graph, c->pb.fuse_p_update, c->pb.cg_form ? 2 : 1, prop.name,
+        prop.multiProcessorCount);
     if (len > 0) {
         std::strncpy(buf, tmp, (size_t)len - 1);
         buf[len - 1] = '\0';

@@ -93,6 +93,10 @@ struct Scalars {
     int sweep1_steps;  // mixed solve: inner steps of the first two sweeps
     int sweep2_steps;
     int pad;
+    // one-reduction CG form: rho and alpha of CG step k in slot k & 1 (a step
+    // reads the previous step's slot while its workgroup 0 writes its own)
+    double cg_rho[2];
+    double cg_alpha[2];
 };
 
 // Sum of this workgroup's per-thread values, written to part[blockIdx.x]

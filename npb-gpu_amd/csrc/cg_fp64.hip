@@ -425,6 +425,86 @@ __global__ __launch_bounds__(kBlock) void update_p_kernel(
     }
 }
 
+// One CG step of the one-reduction form (Chronopoulos and Gear, 1989), after
+// s = A r (the SELL SpMV of r, its block row partials and r.s partials):
+//   beta = rho / rho_prev, p = r + beta p, q = s + beta q (= A p),
+//   delta = p.q = r.s - beta rho / alpha_prev, alpha = rho / delta,
+//   z += alpha p, r -= alpha q, partials of the new r.r.
+// It is the reference's step (cg.cu:703-760) with p.q taken from r.s and the
+// previous step's scalars (r_k orthogonal to r_(k-1), A symmetric), so a step
+// is two launches; the first step has beta = 0 (p = r = x, q = s).  rho_prev
+// and alpha_prev sit in slot (step - 1) & 1 of the scalars; workgroup 0
+// writes this step's into slot step & 1.
+__global__ __launch_bounds__(kBlock) void update_cg_kernel(
+    double *__restrict__ p, double *__restrict__ q, double *__restrict__ z,
+    double *__restrict__ r, int n, const double *__restrict__ rho_part, int n_rho,
+    const double *__restrict__ rs_part, int n_rs, double *__restrict__ rr_out,
+    const double *__restrict__ rows_part, int nblk, Scalars *sc, int step) {
+    // this thread's first row, loaded ahead of the reductions
+    const int j0 = blockIdx.x * kBlock + threadIdx.x;
+    constexpr int kPre = 8;
+    double sb[kPre], p0 = 0.0, q0 = 0.0, z0 = 0.0, r0 = 0.0;
+    if (j0 < n) {
+#pragma unroll
+        for (int b = 0; b < kPre; b++)
+            if (b == 0 || b < nblk) sb[b] = rows_part[(size_t)b * n + j0];
+        p0 = p[j0];
+        q0 = q[j0];
+        z0 = z[j0];
+        r0 = r[j0];
+    }
+    double rho, rs;
+    reduce_partials2<kBlock>(rho_part, n_rho, rs_part, n_rs, rho, rs);
+    double beta = 0.0, delta = rs;
+    if (step > 0) {
+        const double rho_prev = sc->cg_rho[(step - 1) & 1];
+        const double alpha_prev = sc->cg_alpha[(step - 1) & 1];
+        beta = rho / rho_prev;
+        delta = rs - beta * rho / alpha_prev;
+    }
+    const double alpha = rho / delta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->cg_rho[step & 1] = rho;
+        sc->cg_alpha[step & 1] = alpha;
+        sc->rho0 = rho;
+        sc->beta = beta;
+        sc->d = delta;
+        sc->alpha = alpha;
+        sc->steps += 1;
+    }
+    double acc = 0.0;
+    for (int j = j0; j < n; j += gridDim.x * kBlock) {
+        double sj, pj, qj, zj, rj;
+        if (j == j0) {
+            sj = sb[0];
+#pragma unroll
+            for (int b = 1; b < kPre; b++)
+                if (b < nblk) sj += sb[b];
+            for (int b = kPre; b < nblk; b++) sj += rows_part[(size_t)b * n + j];
+            pj = p0;
+            qj = q0;
+            zj = z0;
+            rj = r0;
+        } else {
+            sj = rows_part[j];
+            for (int b = 1; b < nblk; b++) sj += rows_part[(size_t)b * n + j];
+            pj = p[j];
+            qj = q[j];
+            zj = z[j];
+            rj = r[j];
+        }
+        pj = rj + beta * pj;
+        qj = sj + beta * qj;
+        p[j] = pj;
+        q[j] = qj;
+        z[j] = zj + alpha * pj;
+        const double rn = rj - alpha * qj;
+        r[j] = rn;
+        acc += rn * rn;
+    }
+    store_block_partial<kBlock>(acc, rr_out);
+}
+
 // Start of conj_grad (cg.cu:671-686): q = z = 0, r = p = x for j <= n and
 // partials of x.x over j < n.  With scale_x, x = xscale * z (j < n) is
 // applied first: the normalisation of the previous outer step (:479-481).
@@ -619,7 +699,8 @@ hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEve
                         (const double *)pb.part.rr[(step - 1) & 1],
                         rho_source(pb, step - 1), pb.vec_blocks, (const Scalars *)pb.sc, rev);
     }
-    const double *v = MODE == 0 ? p_of_step(pb, step) : pb.z;
+    // one-reduction form: the CG SpMV is s = A r (its dot partials are r.s)
+    const double *v = MODE == 0 ? (pb.cg_form ? pb.r : p_of_step(pb, step)) : pb.z;
     return launch_k(spmv_sell_kernel<BLOCK, 8, MODE, false>, d.grid, BLOCK, d.lds_bytes, st,
                     e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v, d.rows_part,
                     pb.part.pq, pb.n, d.W, d.nrange, (const double *)nullptr,
@@ -680,11 +761,15 @@ int spmv_rows_per_block(int variant) {
 }
 
 hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st) {
-    return launch_spmv<0>(pb, st);
+    Fp64Problem pq = pb;
+    pq.cg_form = 0;  // q = A p, whatever form the CG loop uses
+    return launch_spmv<0>(pq, st);
 }
 
 hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st) {
-    hipError_t e = launch_spmv<0>(pb, st);
+    Fp64Problem pq = pb;
+    pq.cg_form = 0;  // q = A p, whatever form the CG loop uses
+    hipError_t e = launch_spmv<0>(pq, st);
     if (e != hipSuccess || pb.spmv_variant != kSpmvTiled) return e;
     combine_rows_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.sell.rows_part, pb.sell.nblk,
                                                           pb.q, pb.n);
@@ -782,6 +867,24 @@ hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream
 
 hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, int record,
                              hipStream_t st, const SpmvProbe *probe) {
+    if (pb.cg_form) {
+        // one-reduction form: s = A r, then update_cg, two launches a step;
+        // a last p = r + beta p leaves p as the reference's loop does
+        hipError_t e = launch_cg_init(pb, scale_x, st);
+        for (int it = 0; it < cgitmax && e == hipSuccess; it++) {
+            const bool timed = probe && probe->start && it < probe->count;
+            e = launch_spmv<0>(pb, st, timed ? probe->start[it] : nullptr,
+                               timed ? probe->stop[it] : nullptr, it);
+            if (e != hipSuccess) break;
+            update_cg_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+                pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, it), pb.vec_blocks, pb.part.pq,
+                spmv_grid(pb), pb.part.rr[it & 1], pb.sell.rows_part, pb.sell.nblk, pb.sc, it);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && cgitmax > 0) e = launch_update_p(pb, cgitmax - 1, st);
+        if (e == hipSuccess) e = launch_resid(pb, record, st);
+        return e;
+    }
     // With the fused SELL path, p = r + beta p of step it-1 is formed inside
     // step it's SpMV, so a step is two launches; one update_p closes the last.
     const bool fused = fused_p(pb);

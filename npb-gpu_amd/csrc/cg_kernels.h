@@ -65,6 +65,8 @@ struct Fp64Problem {
     int spmv_blocks;            // workgroups of the CSR SpMV kernels
     int spmv_variant;
     int fuse_p_update;          // SELL path: p update folded into the next SpMV
+    int cg_form;                // SELL path, conj_grad only: 1 = one-reduction form
+                                // (s = A r, then one update kernel per step)
     const double *minv;         // diagonal preconditioner M^-1 (nullptr: plain CG)
     const PrecondOp *pop;       // sparse preconditioner (nullptr: none); z = M^-1 r in pz
     double *pz;

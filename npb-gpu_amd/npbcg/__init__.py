@@ -48,7 +48,8 @@ class Opts(ctypes.Structure):
                 ("engine", ctypes.c_int), ("spmv_variant", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("tile_rows", ctypes.c_int),
                 ("tile_width", ctypes.c_int), ("fuse_p_update", ctypes.c_int),
-                ("solve_batch", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+                ("solve_batch", ctypes.c_int), ("cg_form", ctypes.c_int),
+                ("reserved", ctypes.c_int * 6)]
 
 
 SOLVE_FP64, SOLVE_MIXED = 0, 1

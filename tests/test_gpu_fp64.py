@@ -248,3 +248,38 @@ def test_single_system_tuner_persists_and_matches_oracle(gpu, golden, tmp_path, 
         zeta, rnorm = ctx.power(niter, shift)
     ozeta, _, _ = oracle.power(_ocsr(m), niter)
     _zeta_close(zeta, ozeta, f"{size} tuned shape vs oracle")
+
+
+@pytest.mark.parametrize("cg_form", [1, 2])
+@pytest.mark.parametrize("name", ["S_npb", "A_npb", "B_npb", "cfg3_n150000_seed314"])
+def test_cg_forms_match_reference(gpu, golden, name, cg_form):
+    # cg_form 1: the reference's step order; 2: the one-reduction form (s = A r,
+    # p.q from r.s and the previous step's scalars).  Both are held to the
+    # reference's own zeta and rnorm trajectories.
+    c = golden[name]
+    m = _gen(gpu, c)
+    with gpu.Context(m, spmv_variant=5, cg_form=cg_form) as ctx:
+        assert ctx.describe()["cg_form"] == cg_form
+        zeta, rnorm = ctx.power(c["niter"], c["shift"])
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"]], f"{name} form {cg_form}")
+    rref = np.array([float.fromhex(h) for h in c["rnorm_hex"]])
+    assert np.all(np.abs(rnorm - rref) <= 1e-12 * np.sqrt(m.n))
+
+
+@pytest.mark.parametrize("cg_form", [1, 2])
+def test_cg_forms_dropin_vectors(gpu, golden, cg_form):
+    c = golden["cfg1_n1400_seed12345"]
+    m = _gen(gpu, c)
+    n = m.n
+    rng = np.random.default_rng(11)
+    x = np.ones(n + 2)
+    x[:n] += 0.25 * rng.standard_normal(n)
+    x[n] = 2.0
+    with gpu.Context(m, spmv_variant=5, cg_form=cg_form) as ctx:
+        z, p, q, r, rn = ctx.conj_grad(x)
+    oz, op, oq, orr, orn = oracle.conj_grad(_ocsr(m), x)
+    xmax = float(np.max(np.abs(x[:n])))
+    for v, ref, lab in [(z, oz, "z"), (p, op, "p"), (q, oq, "q"), (r, orr, "r")]:
+        _vec_close(v[:n], ref[:n], xmax, f"form {cg_form} {lab}")
+    assert abs(rn - orn) <= 1e-12 * np.linalg.norm(x[:n])
+    assert z[n] == 0.0 and q[n] == 0.0 and r[n] == 2.0 and p[n] == 2.0
