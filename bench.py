@@ -12,8 +12,10 @@ bytes of one outer iteration = 26 * B_spmv + 25 * 72 n + 56 n, where
 B_spmv = 12 nnz + 4 (n+1) + 16 n (matrix streamed once, vector read once,
 result written once).  The SpMV kernel's own duration is
 sampled with HIP events on the library's stream during the timed steps and
-gives the roofline entry.  cpu_baseline: the oracle's serial power loop
-(single thread) on a bounded sample of the same workload.
+gives the roofline entry.  cpu_baseline: the reference's serial power loop
+(its own code, built under oracle/_ref from the reference sources; the
+oracle's C restatement where that build is absent), single thread, on the
+same matrix.
 """
 from __future__ import annotations
 
@@ -221,11 +223,11 @@ def pmc_summary(fetch_csv, write_csv, out_path):
            "fetch_scale": f_scale, "write_scale": w_scale,
            "spmv_fetch_bytes": spmv_f, "spmv_write_bytes": spmv_w,
            "bytes_per_launch": spmv_f + spmv_w,
-           "workload": "n=75000 nonzer=13 shift=60 seed=271828, FP64, SELL column-blocked SpMV q = A p",
+           "workload": "n=75000 nonzer=13 shift=60 seed=271828, FP64, SELL column-blocked CG SpMV",
            "commands": [
-               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof_fetch -o fetch "
+               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d prof_fetch -o fetch "
                "-- python3 bench.py --profile-pass",
-               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof_write -o write "
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d prof_write -o write "
                "-- python3 bench.py --profile-pass",
                "python3 bench.py --pmc-summary <fetch csv> <write csv> " + out_path],
            "note": "FETCH_SIZE/WRITE_SIZE are L2-to-fabric KiB per dispatch; each is divided by "

@@ -760,12 +760,6 @@ int spmv_rows_per_block(int variant) {
     }
 }
 
-hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st) {
-    Fp64Problem pq = pb;
-    pq.cg_form = 0;  // q = A p, whatever form the CG loop uses
-    return launch_spmv<0>(pq, st);
-}
-
 hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st) {
     Fp64Problem pq = pb;
     pq.cg_form = 0;  // q = A p, whatever form the CG loop uses

@@ -89,9 +89,6 @@ int spmv_rows_per_block(int variant);
 // Raises the dynamic-LDS limit of the SELL SpMV kernels to lds_bytes.
 hipError_t prepare_sell_kernels(int lds_bytes);
 
-// q = A p (partials p.q).  With the SELL layout q itself is assembled later,
-// by update_zr, from the per-block row sums.
-hipError_t launch_spmv_pq(const Fp64Problem &pb, hipStream_t st);
 // A complete q = A p for every layout (SELL: SpMV then block combine).
 hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st);
 // r = A z (partials ||x - r||^2, x.z, z.z), then a one-workgroup finish that
