@@ -423,6 +423,7 @@ __global__ __launch_bounds__(kBlock) void batch_resid_combine_kernel(BatchDev bd
         double ax[NR];
 #pragma unroll
         for (int k = 0; k < NR; k++) ax[k] = sp[local + k];
+#pragma unroll 8
         for (int blk = 1; blk < nb; blk++)
 #pragma unroll
             for (int k = 0; k < NR; k++) ax[k] += sp[(size_t)blk * ns * NR + local + k];
@@ -513,6 +514,7 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(BatchDev bd, const
             const size_t local = (size_t)(row - srow) * NR;
 #pragma unroll
             for (int k = 0; k < NR; k++) qv[k] = part[local + k];
+#pragma unroll 8
             for (int b = 1; b < nb; b++)
 #pragma unroll
                 for (int k = 0; k < NR; k++) qv[k] += part[(size_t)b * ns * NR + local + k];

@@ -452,15 +452,35 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
         }
         W = std::max(2, (W + 1) & ~1);
         if (W > n) W = (n + 1) & ~1;
-        const int block = (W * 8 <= lds_budget / 2 - 1024) ? 512 : 1024;
-        const int nblk = (n + W - 1) / W;
-        int R;
-        if (c->opts.tile_rows > 0) {
-            R = c->opts.tile_rows;
-        } else {
+        int block = (W * 8 <= lds_budget / 2 - 1024) ? 512 : 1024;
+        int nblk = (n + W - 1) / W;
+        auto default_rows = [&]() {
             const int slots = cus * (block == 512 ? 2 : 1);
             const int nrange = std::max(1, slots / nblk);
-            R = (n + nrange - 1) / nrange;
+            return (int)(((long long)n + nrange - 1) / nrange);
+        };
+        int R = c->opts.tile_rows > 0 ? c->opts.tile_rows : default_rows();
+        // The per-slice p.q slots (one double per 64-row slice of a workgroup)
+        // share the LDS with the vector block.  When the range's slices do not
+        // fit beside the block (large n: few ranges of many rows), the block
+        // is narrowed to 7/8 of the LDS and the range capped at the slices
+        // the rest holds: more, shorter workgroups instead of the CSR kernel.
+        const int kSlice = npbcg_detail::kSliceRows;
+        if ((long long)(W + (R + kSlice - 1) / kSlice) * 8 > lds_budget) {
+            if (c->opts.tile_width <= 0) {
+                const int wcap = (lds_budget / 8) * 7 / 8;
+                if (W > wcap) {
+                    const int nb = (n + wcap - 1) / wcap;
+                    W = std::max(2, ((n + nb - 1) / nb + 1) & ~1);
+                    block = (W * 8 <= lds_budget / 2 - 1024) ? 512 : 1024;
+                    nblk = (n + W - 1) / W;
+                    if (c->opts.tile_rows <= 0) R = default_rows();
+                }
+            }
+            if (c->opts.tile_rows <= 0) {
+                const int rmax = kSlice * (lds_budget / 8 - W - 1);
+                if (rmax >= kSlice) R = std::min(R, rmax);
+            }
         }
         if (W * 8 > lds_budget || !sorted ||
             !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, &sm)) {

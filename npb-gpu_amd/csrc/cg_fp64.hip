@@ -269,6 +269,7 @@ __global__ __launch_bounds__(kBlock) void combine_rows_kernel(
     const double *__restrict__ rows_part, int nblk, double *__restrict__ y, int n) {
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         double s = rows_part[j];
+#pragma unroll 8
         for (int b = 1; b < nblk; b++) s += rows_part[(size_t)b * n + j];
         y[j] = s;
     }
@@ -284,6 +285,7 @@ __global__ __launch_bounds__(kBlock) void resid_combine_kernel(
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         double rj = rows_part[j];
+#pragma unroll 8
         for (int b = 1; b < nblk; b++) rj += rows_part[(size_t)b * n + j];
         r[j] = rj;
         const double xv = x[j], zv = z[j];
@@ -358,6 +360,7 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
 #pragma unroll
             for (int b = 1; b < kPre; b++)
                 if (b < nblk) qj += qb[b];
+#pragma unroll 8
             for (int b = kPre; b < nblk; b++) qj += rows_part[(size_t)b * n + j0];
             q[j0] = qj;
         } else {
@@ -372,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
         double qj;
         if (nblk > 0) {
             qj = rows_part[j];
+#pragma unroll 8
             for (int b = 1; b < nblk; b++) qj += rows_part[(size_t)b * n + j];
             q[j] = qj;
         } else {
@@ -480,6 +484,7 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
 #pragma unroll
             for (int b = 1; b < kPre; b++)
                 if (b < nblk) sj += sb[b];
+#pragma unroll 8
             for (int b = kPre; b < nblk; b++) sj += rows_part[(size_t)b * n + j];
             pj = p0;
             qj = q0;
@@ -487,6 +492,7 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
             rj = r0;
         } else {
             sj = rows_part[j];
+#pragma unroll 8
             for (int b = 1; b < nblk; b++) sj += rows_part[(size_t)b * n + j];
             pj = p[j];
             qj = q[j];

@@ -233,6 +233,7 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
 #pragma unroll
         for (int b = 1; b < kPre; b++)
             if (b < nblk) qj += qb[b];
+#pragma unroll 8
         for (int b = kPre; b < nblk; b++) qj += rows_part[(size_t)b * n + j0];
         q[j0] = qj;
         d[j0] = fmaf(a32, p0, d0);
@@ -242,6 +243,7 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
     }
     for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock) {
         float qj = rows_part[j];
+#pragma unroll 8
         for (int b = 1; b < nblk; b++) qj += rows_part[(size_t)b * n + j];
         q[j] = qj;
         d[j] = fmaf(a32, p[j], d[j]);

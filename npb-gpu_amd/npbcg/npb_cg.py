@@ -63,13 +63,14 @@ def cpu_model() -> str:
 def run(cls: str | None, n: int, nonzer: int, niter: int, shift: float, seed: float,
         device: int = 0, out=sys.stdout) -> dict:
     verify = CLASSES[cls][4] if cls else None
-    p = lambda *a: print(*a, file=out)  # noqa: E731
+    p = lambda *a: print(*a, file=out, flush=True)  # noqa: E731
     p("\n\n NAS Parallel Benchmarks 4.1 npbcg (gfx950 HIP) version - CG Benchmark\n")
     p(f" Size: {n:11d}")
     p(f" Iterations: {niter:5d}")
     t0 = time.perf_counter()
     m = npbcg.makea(n, nonzer, shift, seed, RCOND)
     t_init = time.perf_counter() - t0
+    p(f" Initialization time = {t_init:15.3f} seconds")
     with npbcg.Context(m, device=device) as ctx:
         ctx.fill_x(1.0)
         ctx.power(1, shift)          # untimed warm-up iteration (cg.cu:458-482)
