@@ -26,10 +26,16 @@ void parallel_items(int count, F &&body) {
 }
 
 // Rows of range rr ordered by their segment length in block b (longest
-// first, ties by row), with the segment bounds.
+// first, ties by row) within windows of kSortWindow rows, with the segment
+// bounds.  The window keeps the rows of one slice, whose block sums are
+// stored as 8-byte partials, within a few thousand rows of each other, so
+// the partial stores of large ranges stay in L2-sized neighbourhoods.  It
+// is a multiple of the slice height, so a slice never spans two windows.
 struct RowSeg {
     int row, lo, hi;
 };
+constexpr int kSortWindow = 2048;
+static_assert(kSortWindow % kSliceRows == 0, "window of whole slices");
 
 void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, int b,
                     int rr, std::vector<RowSeg> *out) {
@@ -42,9 +48,13 @@ void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, i
         const int hi = (int)(std::lower_bound(first, last, c1) - colidx);
         out->push_back({j, lo, hi});
     }
-    std::stable_sort(out->begin(), out->end(), [](const RowSeg &x, const RowSeg &y) {
-        return (x.hi - x.lo) > (y.hi - y.lo);
-    });
+    for (size_t w0 = 0; w0 < out->size(); w0 += kSortWindow) {
+        const size_t w1 = std::min(out->size(), w0 + (size_t)kSortWindow);
+        std::stable_sort(out->begin() + w0, out->begin() + w1,
+                         [](const RowSeg &x, const RowSeg &y) {
+                             return (x.hi - x.lo) > (y.hi - y.lo);
+                         });
+    }
 }
 
 template <class V>

@@ -3,9 +3,10 @@
 // Columns are cut into nblk blocks of W columns and rows into nrange ranges
 // of R rows.  Workgroup wg = b * nrange + rr owns the part of row range rr that
 // falls into column block b.  Inside a workgroup the rows are ordered by their
-// segment length in block b (longest first, ties by row) and cut into slices
-// of 64 rows; slice s stores its entries column-major: entry j of the slice's
-// lane l sits at slice_off[s] + 64*j + l, and rows shorter than the slice
+// segment length in block b (longest first, ties by row) within windows of
+// 2048 rows, and cut into slices of 64 rows; slice s stores its entries
+// column-major: entry j of the slice's lane l sits at slice_off[s] + 64*j + l,
+// and rows shorter than the slice
 // width are padded with (value 0, column 0).  So one wavefront streams a slice
 // with fully coalesced loads, one row per lane, and no cross-lane reduction.
 // Column indices are 16-bit offsets inside the block (W <= 65536).
