@@ -81,7 +81,8 @@ struct BatchVecs {
     double *part_init = nullptr, *part_pq = nullptr, *part_rr[2] = {nullptr, nullptr};
 };
 
-// Step `step` of an inner solve of every unfinished system (3 launches + publish).
+// Step `step` of an inner solve of every unfinished system: SpMM + p.q,
+// x/r update + r.r, p update + stop test, and the flag publish.
 template <typename T>
 hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, int first,
                          hipStream_t st);

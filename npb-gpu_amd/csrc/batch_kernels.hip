@@ -7,11 +7,14 @@
 // rows that never cross a system boundary, one workgroup per tile, so every
 // per-system reduction is a fixed-order sum over that system's tile partials
 // and no atomics are needed.  One CG step for the whole batch is three
-// launches (SpMM + p.q, x/r update + r.r, p update + stop test); a system whose
-// right-hand sides have all converged costs its tiles one flag read.
+// launches (SpMM + p.q, x/r update + r.r, p update + stop test) and a flag
+// publish; a system whose right-hand sides have all converged costs its
+// workgroups one flag read.
 //
-// The SpMM reads each matrix entry once for all NR vectors: LPR lanes share a
-// row, every lane gathers the NR values of its column as one vector load.
+// The SpMM reads each matrix entry once for all NR vectors.  Three layouts:
+// row gathers (LPR lanes share a CSR row, every lane gathers the NR values of
+// its column as one vector load), column blocks in CSR order, and SELL slices
+// over LDS-staged column blocks (one lane per row; the default).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -3,11 +3,15 @@
 // One inner CG iteration of the reference (CUDA/CG/cg.cu:814-837) is six
 // launches and two blocking device-to-host copies there (k3 SpMV, k4 p.q,
 // k5 two axpys, k6 r.r, k7 p update, with alpha/beta formed on the host).
-// Here it is three launches and nothing returns to the host:
+// Here nothing returns to the host.  In the reference's step order it is
+// three launches:
 //   spmv       q = A p, per-workgroup partials of p.q                 (k3+k4)
 //   update_zr  reduces rho and p.q partials -> alpha; z += alpha p,
 //              r -= alpha q; partials of the new r.r                  (k5+k6)
 //   update_p   reduces old and new r.r -> beta; p = r + beta p        (k7)
+// In the one-reduction form (conj_grad's default on the SELL layout) it is
+// two: spmv of r (s = A r, partials of r.s) and update_cg, which forms beta,
+// p, q = A p by recursion, p.q from r.s, alpha, z and r (k3..k7).
 // Around the inner loop:
 //   cg_init    optionally x = xscale z (the previous outer step's
 //              normalisation); q = z = 0, r = p = x through index n;
