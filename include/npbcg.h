@@ -155,6 +155,28 @@ int npbcg_fill_x(npbcg_ctx *ctx, double value);
 int npbcg_set_spmv_sampling(npbcg_ctx *ctx, int every_nth_call);
 int npbcg_get_timing(const npbcg_ctx *ctx, npbcg_timing *out);
 
+/* Device time of one kernel of the path, summed over a profiled run. */
+typedef struct {
+    char name[64];   /* kernel (and the product it forms, for the SpMV) */
+    int threads;     /* workgroup size */
+    int calls;       /* launches */
+    double ms;       /* total device time */
+} npbcg_kernel_time;
+
+/* Profiled npbcg_power: the same niter outer iterations from the current x
+ * (same results; zeta_out nullable), launched kernel by kernel without the
+ * captured graph, with a HIP event recorded after every launch.  Each
+ * launch's time is the gap between its event and the one before, so the
+ * kernel times add up to *total_ms, the whole run.  out[0..*nkinds) receives
+ * one entry per kernel in order of first launch (at most max_kinds; a larger
+ * count returns NPBCG_ERR_INVALID).  Replaces the reference's PROFILING build
+ * (cg.cu:83-94 timers around each launch, table printed at cg.cu:565-589).
+ * Library calls also open roctx ranges named after the call
+ * ("npbcg_power", "npbcg_solve", ...) for rocprofv3 --marker-trace. */
+int npbcg_profile_kernels(npbcg_ctx *ctx, int niter, double shift, double *zeta_out,
+                          npbcg_kernel_time *out, int max_kinds, int *nkinds,
+                          double *total_ms);
+
 /* The context's kernel configuration as a one-line JSON object (SpMV variant,
  * block/tile shape, workgroup counts, stored entries) written to buf
  * (NUL-terminated, truncated to len).  Returns the full length. */

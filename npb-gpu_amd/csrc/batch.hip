@@ -13,6 +13,7 @@
 #include "batch_kernels.h"
 #include "npbcg.h"
 #include "npbcg_internal.h"
+#include "trace_range.h"
 #include "tiled_layout.h"
 
 using namespace npbcg_dev;
@@ -528,6 +529,7 @@ void npbcg_batch_destroy(npbcg_batch *b) {
 int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *const *rowstr,
                        const int *const *colidx, const double *const *a,
                        const npbcg_batch_opts *opts) {
+    npbcg_detail::TraceRange trace("npbcg_batch_create");
     if (!out || nsys < 1 || !n || !rowstr || !colidx || !a)
         return set_error(NPBCG_ERR_INVALID, "npbcg_batch_create: invalid argument");
     *out = nullptr;
@@ -829,6 +831,7 @@ int npbcg_batch_set_rhs(npbcg_batch *b, const double *rhs) {
 
 int npbcg_batch_tune(npbcg_batch *b, const char *cache_path, int *lanes_per_row,
                      int *rows_per_tile, double *ms_out) {
+    npbcg_detail::TraceRange trace("npbcg_batch_tune");
     if (!b) return set_error(NPBCG_ERR_INVALID, "npbcg_batch_tune: NULL");
     B_HIP(hipSetDevice(b->device));
     const std::string key = shape_key(b);
@@ -876,6 +879,7 @@ int npbcg_batch_tune(npbcg_batch *b, const char *cache_path, int *lanes_per_row,
 
 int npbcg_batch_solve(npbcg_batch *b, double rtol, int maxit, npbcg_batch_stats *stats,
                       int *steps_out, double *relres_out) {
+    npbcg_detail::TraceRange trace("npbcg_batch_solve");
     if (!b || !(rtol > 0.0) || maxit < 1)
         return set_error(NPBCG_ERR_INVALID, "npbcg_batch_solve: invalid argument");
     if (!b->rhs_set) return set_error(NPBCG_ERR_INVALID, "npbcg_batch_solve: no right-hand sides");

@@ -17,6 +17,7 @@
 #include "learned_kernels.h"
 #include "npbcg.h"
 #include "npbcg_internal.h"
+#include "trace_range.h"
 #include "tiled_layout.h"
 
 using npbcg_dev::Fp64Problem;
@@ -377,6 +378,7 @@ const char *npbcg_status_string(int s) {
 
 int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
                  const double *a, const npbcg_opts *opts) {
+    npbcg_detail::TraceRange trace("npbcg_create");
     if (!out) return set_error(NPBCG_ERR_INVALID, "npbcg_create: out is NULL");
     *out = nullptr;
     if (n < 1 || !rowstr || !colidx || !a)
@@ -626,6 +628,7 @@ void npbcg_destroy(npbcg_ctx *c) {
 }
 
 int npbcg_conj_grad_device(npbcg_ctx *c, double *rnorm) {
+    npbcg_detail::TraceRange trace("npbcg_conj_grad");
     if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_conj_grad_device: NULL context");
     NPBCG_HIP(hipSetDevice(c->device));
     NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
@@ -639,7 +642,7 @@ int npbcg_conj_grad_device(npbcg_ctx *c, double *rnorm) {
     c->timing.total_ms = ms;
     c->timing.spmv_samples = 0;
     c->timing.spmv_avg_ms = 0.0;
-    c->timing.launches = 3 * c->cgitmax + 3;
+    c->timing.launches = npbcg_dev::conj_grad_launches(c->pb, c->cgitmax);
     if (rnorm) *rnorm = s.rnorm;
     clear_error();
     return NPBCG_OK;
@@ -696,6 +699,7 @@ int npbcg_scale_x(npbcg_ctx *c, double scale) {
 
 int npbcg_power(npbcg_ctx *c, int niter, double shift, double *zeta_out,
                 double *rnorm_out) {
+    npbcg_detail::TraceRange trace("npbcg_power");
     if (!c || niter < 0) return set_error(NPBCG_ERR_INVALID, "npbcg_power: invalid argument");
     NPBCG_HIP(hipSetDevice(c->device));
     int rc = ensure_history(c, std::max(niter, 1));
@@ -722,14 +726,14 @@ int npbcg_power(npbcg_ctx *c, int niter, double shift, double *zeta_out,
         if (sample && it == 0) {
             NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, scale, 1, c->stream,
                                                    &probe));
-            launches += 3 * c->cgitmax + 3;
+            launches += npbcg_dev::conj_grad_launches(c->pb, c->cgitmax);
         } else if (use_graph) {
             NPBCG_HIP(hipGraphLaunch(c->outer_graph[scale], c->stream));
             launches += 1;
         } else {
             NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, scale, 1, c->stream,
                                                    nullptr));
-            launches += 3 * c->cgitmax + 3;
+            launches += npbcg_dev::conj_grad_launches(c->pb, c->cgitmax);
         }
     }
     if (niter > 0) NPBCG_HIP(npbcg_dev::launch_scale_x(c->pb, 0.0, 1, c->stream));
@@ -757,6 +761,60 @@ int npbcg_power(npbcg_ctx *c, int niter, double shift, double *zeta_out,
         c->timing.spmv_samples = probe.count;
         c->timing.spmv_avg_ms = sum / probe.count;
     }
+    clear_error();
+    return NPBCG_OK;
+}
+
+int npbcg_profile_kernels(npbcg_ctx *c, int niter, double shift, double *zeta_out,
+                          npbcg_kernel_time *out, int max_kinds, int *nkinds,
+                          double *total_ms) {
+    npbcg_detail::TraceRange trace("npbcg_profile_kernels");
+    if (!c || niter < 1 || !out || max_kinds < 1 || !nkinds)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_profile_kernels: invalid argument");
+    NPBCG_HIP(hipSetDevice(c->device));
+    int rc = ensure_history(c, niter);
+    if (rc != NPBCG_OK) return rc;
+    NPBCG_HIP(npbcg_dev::launch_reset_scalars(c->pb, shift, c->stream));
+    npbcg_dev::KernelMarks marks;
+    Fp64Problem pb = c->pb;
+    pb.marks = &marks;
+    marks.stamp(c->stream);
+    hipError_t e = marks.err;
+    for (int it = 0; it < niter && e == hipSuccess; it++)
+        e = npbcg_dev::enqueue_conj_grad(pb, c->cgitmax, it > 0 ? 1 : 0, 1, c->stream, nullptr);
+    if (e == hipSuccess) e = npbcg_dev::launch_scale_x(pb, 0.0, 1, c->stream);
+    if (e == hipSuccess) e = marks.err;
+    NPBCG_HIP(e);
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    if (zeta_out)
+        NPBCG_HIP(hipMemcpy(zeta_out, c->d_hist, sizeof(double) * niter, hipMemcpyDeviceToHost));
+    // one entry per kernel name, in order of first launch
+    std::vector<npbcg_kernel_time> kinds;
+    double total = 0.0;
+    for (size_t i = 0; i < marks.name.size(); i++) {
+        float t = 0.f;
+        NPBCG_HIP(hipEventElapsedTime(&t, marks.ev[i], marks.ev[i + 1]));
+        total += t;
+        size_t k = 0;
+        while (k < kinds.size() && std::strcmp(kinds[k].name, marks.name[i]) != 0) k++;
+        if (k == kinds.size()) {
+            npbcg_kernel_time kt{};
+            std::snprintf(kt.name, sizeof(kt.name), "%s", marks.name[i]);
+            kt.threads = marks.threads[i];
+            kinds.push_back(kt);
+        }
+        kinds[k].calls++;
+        kinds[k].ms += t;
+    }
+    *nkinds = (int)kinds.size();
+    if ((int)kinds.size() > max_kinds)
+        return set_error(NPBCG_ERR_INVALID, "npbcg_profile_kernels: more kernels than max_kinds");
+    std::copy(kinds.begin(), kinds.end(), out);
+    if (total_ms) *total_ms = total;
+    c->timing.total_ms = total;
+    c->timing.launches = (int)marks.name.size() + 1;
+    c->timing.spmv_samples = 0;
+    c->timing.spmv_avg_ms = 0.0;
     clear_error();
     return NPBCG_OK;
 }
@@ -914,6 +972,7 @@ extern "C" {
 
 int npbcg_tune(int n, const int *rowstr, const int *colidx, const double *a, npbcg_opts *opts,
                const char *cache_path, double *ms_out) {
+    npbcg_detail::TraceRange trace("npbcg_tune");
     if (n < 1 || !rowstr || !colidx || !a || !opts)
         return set_error(NPBCG_ERR_INVALID, "npbcg_tune: invalid argument");
     const npbcg_opts base = *opts;
@@ -1476,6 +1535,7 @@ int npbcg_graph(npbcg_ctx *c, int *nbr, float *wgt, float *nval, float *feat) {
 }
 
 int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats *st) {
+    npbcg_detail::TraceRange trace("npbcg_solve");
     if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_solve: NULL context");
     NPBCG_HIP(hipSetDevice(c->device));
     int rc = select_precond(c, c->precond);

@@ -666,14 +666,23 @@ hipError_t launch_k(K kernel, int grid, int block, int lds, hipStream_t st, hipE
     return hipGetLastError();
 }
 
+// The launch's error, after a profiling stamp when the run is profiled.
+hipError_t noted(const Fp64Problem &pb, hipError_t e, const char *kname, int threads,
+                 hipStream_t st) {
+    if (e == hipSuccess && pb.marks) pb.marks->note(kname, threads, st);
+    return e;
+}
+
 template <int SEG, int UNR, int MODE>
 hipError_t launch_csr_t(const Fp64Problem &pb, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const double *v = MODE == 0 ? pb.p : pb.z;
     double *y = MODE == 0 ? pb.q : pb.r;
     double *part = MODE == 0 ? pb.part.pq : pb.part.res;
-    return launch_k(spmv_csr_kernel<SEG, UNR, MODE>, pb.spmv_blocks, kBlock, 0, st, e0, e1,
-                    pb.rowstr, pb.colidx, pb.a, v, y, pb.x, pb.n, part,
-                    (const Scalars *)pb.sc);
+    return noted(pb,
+                 launch_k(spmv_csr_kernel<SEG, UNR, MODE>, pb.spmv_blocks, kBlock, 0, st, e0, e1,
+                          pb.rowstr, pb.colidx, pb.a, v, y, pb.x, pb.n, part,
+                          (const Scalars *)pb.sc),
+                 MODE == 0 ? "spmv_csr_kernel (q = A p)" : "spmv_csr_kernel (r = A z)", kBlock, st);
 }
 
 bool fused_p(const Fp64Problem &pb) {
@@ -702,20 +711,28 @@ hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEve
     const SellDev &d = pb.sell;
     const int rev = MODE == 0 ? (step & 1) : 1;
     if (MODE == 0 && step > 0 && fused_p(pb)) {
-        return launch_k(spmv_sell_kernel<BLOCK, 8, 0, true>, d.grid, BLOCK, d.lds_bytes, st,
-                        e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col,
-                        (const double *)p_of_step(pb, step - 1), d.rows_part, pb.part.pq,
-                        pb.n, d.W, d.nrange, (const double *)pb.r, p_of_step(pb, step),
-                        (const double *)pb.part.rr[(step - 1) & 1],
-                        rho_source(pb, step - 1), pb.vec_blocks, (const Scalars *)pb.sc, rev);
+        return noted(pb,
+                     launch_k(spmv_sell_kernel<BLOCK, 8, 0, true>, d.grid, BLOCK, d.lds_bytes,
+                              st, e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col,
+                              (const double *)p_of_step(pb, step - 1), d.rows_part, pb.part.pq,
+                              pb.n, d.W, d.nrange, (const double *)pb.r, p_of_step(pb, step),
+                              (const double *)pb.part.rr[(step - 1) & 1],
+                              rho_source(pb, step - 1), pb.vec_blocks, (const Scalars *)pb.sc,
+                              rev),
+                     "spmv_sell_kernel (p = r + beta p; q = A p)", BLOCK, st);
     }
     // one-reduction form: the CG SpMV is s = A r (its dot partials are r.s)
     const double *v = MODE == 0 ? (pb.cg_form ? pb.r : p_of_step(pb, step)) : pb.z;
-    return launch_k(spmv_sell_kernel<BLOCK, 8, MODE, false>, d.grid, BLOCK, d.lds_bytes, st,
-                    e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v, d.rows_part,
-                    pb.part.pq, pb.n, d.W, d.nrange, (const double *)nullptr,
-                    (double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0,
-                    (const Scalars *)pb.sc, rev);
+    return noted(pb,
+                 launch_k(spmv_sell_kernel<BLOCK, 8, MODE, false>, d.grid, BLOCK, d.lds_bytes,
+                          st, e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v,
+                          d.rows_part, pb.part.pq, pb.n, d.W, d.nrange, (const double *)nullptr,
+                          (double *)nullptr, (const double *)nullptr, (const double *)nullptr,
+                          0, (const Scalars *)pb.sc, rev),
+                 MODE == 1    ? "spmv_sell_kernel (r = A z)"
+                 : pb.cg_form ? "spmv_sell_kernel (s = A r)"
+                              : "spmv_sell_kernel (q = A p)",
+                 BLOCK, st);
 }
 
 template <int MODE>
@@ -786,19 +803,19 @@ hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
     if (pb.spmv_variant == kSpmvTiled) {
         resid_combine_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
             pb.sell.rows_part, pb.sell.nblk, pb.x, pb.z, pb.r, pb.n, pb.part.res, pb.sc);
-        e = hipGetLastError();
+        e = noted(pb, hipGetLastError(), "resid_combine_kernel", kBlock, st);
         if (e != hipSuccess) return e;
     }
     resid_finish_kernel<<<1, kBlock, 0, st>>>(pb.part.res, resid_partials(pb), pb.sc,
                                               pb.zeta_hist, pb.rnorm_hist, pb.hist_cap,
                                               record);
-    return hipGetLastError();
+    return noted(pb, hipGetLastError(), "resid_finish_kernel", kBlock, st);
 }
 
 hipError_t launch_cg_init(const Fp64Problem &pb, int scale_x, hipStream_t st) {
     cg_init_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.p, pb.q, pb.r, pb.n,
                                                      scale_x, pb.sc, pb.part.xx, pb.minv);
-    hipError_t e = hipGetLastError();
+    hipError_t e = noted(pb, hipGetLastError(), "cg_init_kernel", kBlock, st);
     if (e != hipSuccess || !pb.pop) return e;
     // sparse M^-1: p = z = M^-1 r; r.z into rz[1] (read by step 0), r.r stays in xx
     return launch_precond_apply(*pb.pop, pb.r, pb.pz, pb.p, pb.part.rz[1], pb.vec_blocks,
@@ -815,7 +832,7 @@ hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
         spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part,
         pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc,
         (step == 0 && sets_tol) ? 1 : 0, pb.minv, pb.pop ? pb.part.xx : nullptr);
-    return hipGetLastError();
+    return noted(pb, hipGetLastError(), "update_zr_kernel", kBlock, st);
 }
 
 // p = r + beta p closing CG step `step`; the result always lands in pb.p.
@@ -828,7 +845,7 @@ hipError_t launch_update_p(const Fp64Problem &pb, int step, hipStream_t st) {
         update_p_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
             pb.r, p_of_step(pb, step), pb.p, pb.n, pb.part.rr[step & 1], rho_source(pb, step),
             pb.vec_blocks, pb.sc, pb.minv, nullptr);
-    return hipGetLastError();
+    return noted(pb, hipGetLastError(), "update_p_kernel", kBlock, st);
 }
 
 hipError_t launch_norms(const Fp64Problem &pb, hipStream_t st) {
@@ -843,7 +860,7 @@ hipError_t launch_scale_x(const Fp64Problem &pb, double s, int from_scalars,
                           hipStream_t st) {
     scale_x_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.x, pb.z, pb.n, s, from_scalars,
                                                      pb.sc);
-    return hipGetLastError();
+    return noted(pb, hipGetLastError(), "scale_x_kernel", kBlock, st);
 }
 
 hipError_t launch_reset_scalars(const Fp64Problem &pb, double shift, hipStream_t st,
@@ -883,7 +900,7 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
             update_cg_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
                 pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, it), pb.vec_blocks, pb.part.pq,
                 spmv_grid(pb), pb.part.rr[it & 1], pb.sell.rows_part, pb.sell.nblk, pb.sc, it);
-            e = hipGetLastError();
+            e = noted(pb, hipGetLastError(), "update_cg_kernel", kBlock, st);
         }
         if (e == hipSuccess && cgitmax > 0) e = launch_update_p(pb, cgitmax - 1, st);
         if (e == hipSuccess) e = launch_resid(pb, record, st);
@@ -902,6 +919,13 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
     }
     if (e == hipSuccess) e = launch_resid(pb, record, st);
     return e;
+}
+
+int conj_grad_launches(const Fp64Problem &pb, int cgitmax) {
+    const int resid = pb.spmv_variant == kSpmvTiled ? 3 : 2;
+    if (pb.cg_form) return 1 + 2 * cgitmax + (cgitmax > 0 ? 1 : 0) + resid;
+    if (fused_p(pb)) return 1 + 2 * cgitmax + (cgitmax > 0 ? 1 : 0) + resid;
+    return 1 + 3 * cgitmax + resid;
 }
 
 hipError_t enqueue_solve_step(const Fp64Problem &pb, int it, int sets_tol, hipStream_t st) {
@@ -923,5 +947,25 @@ hipError_t enqueue_solve_steps(const Fp64Problem &pb, int steps, int sets_tol,
 }
 
 size_t scalars_bytes() { return sizeof(Scalars); }
+
+void KernelMarks::stamp(hipStream_t st) {
+    if (err != hipSuccess) return;
+    hipEvent_t e = nullptr;
+    err = hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+    if (err != hipSuccess) return;
+    ev.push_back(e);
+    err = hipEventRecord(e, st);
+}
+
+void KernelMarks::note(const char *kname, int nthreads, hipStream_t st) {
+    stamp(st);
+    if (err != hipSuccess) return;
+    name.push_back(kname);
+    threads.push_back(nthreads);
+}
+
+KernelMarks::~KernelMarks() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
 
 }  // namespace npbcg_dev

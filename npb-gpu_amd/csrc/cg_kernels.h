@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace npbcg_dev {
 
 struct Scalars;
@@ -48,6 +50,19 @@ struct Partials {
 
 struct PrecondOp;  // sparse preconditioner operator (learned_kernels.h)
 
+// Events stamped on the stream after each launch of a profiled run: launch
+// i's device time is the gap between stamp i and stamp i + 1 (stamp 0 comes
+// before the first launch).  Used with direct launches only.
+struct KernelMarks {
+    std::vector<hipEvent_t> ev;
+    std::vector<const char *> name;  // kernel of launch i
+    std::vector<int> threads;        // its workgroup size
+    hipError_t err = hipSuccess;
+    void stamp(hipStream_t st);
+    void note(const char *kname, int nthreads, hipStream_t st);
+    ~KernelMarks();
+};
+
 // Device pointers and launch shapes of one FP64 problem.
 struct Fp64Problem {
     int n;
@@ -71,6 +86,7 @@ struct Fp64Problem {
     const PrecondOp *pop;       // sparse preconditioner (nullptr: none); z = M^-1 r in pz
     double *pz;
     SellDev sell;
+    KernelMarks *marks = nullptr;  // profiled run: a stamp after every launch
 };
 
 // Workgroups launched by the SpMV of this problem (= partials it writes).
@@ -120,6 +136,8 @@ hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
 // (spmv, update_zr, update_p), residual SpMV and finish.
 hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, int record,
                              hipStream_t st, const SpmvProbe *probe);
+// Kernel launches of one enqueue_conj_grad.
+int conj_grad_launches(const Fp64Problem &pb, int cgitmax);
 
 // `steps` CG steps (SpMV, update_zr, update_p) of a solve on pb (x holds b,
 // z the iterate); the stop flag makes steps after convergence return at once.

@@ -26,6 +26,7 @@
 #include "learned_kernels.h"
 #include "npbcg.h"
 #include "npbcg_internal.h"
+#include "trace_range.h"
 
 using namespace npbcg_dev;
 using npbcg_detail::clear_error;
@@ -368,6 +369,7 @@ int npbcg_net_loss_grad(npbcg_net *net, npbcg_ctx *ctx, double *loss, float *gra
 }
 
 int npbcg_net_step(npbcg_net *net, npbcg_ctx *ctx, float lr, double *loss) {
+    npbcg_detail::TraceRange trace("npbcg_net_step");
     if (!net || !ctx || !(lr > 0.0f)) return set_error(NPBCG_ERR_INVALID, "npbcg_net_step: invalid argument");
     if (npbcg_detail::ctx_device(ctx) != net->device)
         return set_error(NPBCG_ERR_INVALID, "npbcg_net_step: context on another device");
@@ -385,6 +387,7 @@ int npbcg_net_step(npbcg_net *net, npbcg_ctx *ctx, float lr, double *loss) {
 }
 
 int npbcg_net_apply(npbcg_net *net, npbcg_ctx *ctx, double *ms) {
+    npbcg_detail::TraceRange trace("npbcg_net_apply");
     if (!net || !ctx) return set_error(NPBCG_ERR_INVALID, "npbcg_net_apply: NULL");
     if (npbcg_detail::ctx_device(ctx) != net->device)
         return set_error(NPBCG_ERR_INVALID, "npbcg_net_apply: context on another device");
@@ -415,6 +418,7 @@ int npbcg_net_apply(npbcg_net *net, npbcg_ctx *ctx, double *ms) {
 
 int npbcg_net_train(npbcg_net *net, const npbcg_train_opts *opts, npbcg_train_stats *stats,
                     double *loss_hist, double *base_hist) {
+    npbcg_detail::TraceRange trace("npbcg_net_train");
     if (!net || !opts) return set_error(NPBCG_ERR_INVALID, "npbcg_net_train: NULL");
     const npbcg_train_opts o = *opts;
     if (o.systems < 1 || o.n_min < 64 || o.n_max < o.n_min || o.nonzer_min < 1 ||

@@ -119,6 +119,11 @@ class Timing(ctypes.Structure):
                 ("nnz", ctypes.c_longlong), ("n", ctypes.c_int), ("engine", ctypes.c_int)]
 
 
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("threads", ctypes.c_int), ("calls", ctypes.c_int),
+                ("ms", ctypes.c_double)]
+
+
 def _load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libnpbcg.so is not built ({LIB_PATH}); run __graft_entry__.build()")
@@ -147,6 +152,9 @@ def _load() -> ctypes.CDLL:
         "npbcg_fill_x": (ctypes.c_int, [vp, ctypes.c_double]),
         "npbcg_set_spmv_sampling": (ctypes.c_int, [vp, ctypes.c_int]),
         "npbcg_get_timing": (ctypes.c_int, [vp, ctypes.POINTER(Timing)]),
+        "npbcg_profile_kernels": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, _F64P,
+                                                 ctypes.POINTER(KernelTime), ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_int), _DP]),
         "npbcg_spmv_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
         "npbcg_tune": (ctypes.c_int, [ctypes.c_int, _I32P, _I32P, _F64P, ctypes.c_void_p,
                                       ctypes.c_char_p, _DP]),
@@ -332,6 +340,22 @@ class Context:
         rnorm = np.zeros(max(niter, 1))
         _check(LIB.npbcg_power(self._h, niter, shift, zeta, rnorm), "npbcg_power")
         return zeta[:niter], rnorm[:niter]
+
+    def profile_kernels(self, niter: int, shift: float, max_kinds: int = 32):
+        """Per-kernel device times of niter outer iterations (npbcg_profile_kernels).
+
+        Returns (kernels, total_ms, zeta): kernels is a list of dicts
+        {name, threads, calls, ms} in order of first launch."""
+        zeta = np.zeros(niter)
+        kt = (KernelTime * max_kinds)()
+        nk = ctypes.c_int(0)
+        total = ctypes.c_double(0.0)
+        _check(LIB.npbcg_profile_kernels(self._h, niter, shift, zeta, kt, max_kinds,
+                                         ctypes.byref(nk), ctypes.byref(total)),
+               "npbcg_profile_kernels")
+        kernels = [{"name": kt[i].name.decode(), "threads": kt[i].threads,
+                    "calls": kt[i].calls, "ms": kt[i].ms} for i in range(nk.value)]
+        return kernels, total.value, zeta
 
     def set_vector(self, which: int, v: np.ndarray):
         v = np.ascontiguousarray(v[: self.n + 1], dtype=np.float64)

@@ -12,6 +12,10 @@ loop npbcg_power — and prints the same lines: the per-iteration
 class's published zeta with epsilon 1e-10 (cg.cu:534-552), and the
 print_results summary (c_print_results.cpp:55-98) with Mop/s from the
 reference's operation count (cg.cu:553-558).  --json adds one JSON line.
+--profile repeats the NITER iterations from x = 1 kernel by kernel
+(npbcg_profile_kernels) and prints the per-kernel table of the reference's
+PROFILING build (cg.cu:565-589): workgroup size, seconds and share of the
+profiled run's time.
 """
 from __future__ import annotations
 
@@ -60,8 +64,21 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def kernel_table(kernels: list, total_ms: float) -> str:
+    """The PROFILING table of cg.cu:565-589: kernel, threads per block,
+    seconds and percentage of the total (here one row per kernel of the path,
+    with its launch count)."""
+    lines = [f"{'GPU Kernel':>5s}\t{'Threads Per Block':>25s}\t{'Time in Seconds':>25s}"
+             f"\t{'Time in Percentage':>25s}\t{'Launches':>9s}"]
+    for k in kernels:
+        pct = k["ms"] * 100.0 / total_ms if total_ms > 0 else 0.0
+        lines.append(f"{k['name']:>29s}\t{k['threads']:25d}\t{k['ms'] / 1e3:25f}"
+                     f"\t{pct:24.2f}%\t{k['calls']:9d}")
+    return "\n".join(lines)
+
+
 def run(cls: str | None, n: int, nonzer: int, niter: int, shift: float, seed: float,
-        device: int = 0, out=sys.stdout) -> dict:
+        device: int = 0, out=sys.stdout, profile: bool = False, use_graph: bool = True) -> dict:
     verify = CLASSES[cls][4] if cls else None
     p = lambda *a: print(*a, file=out, flush=True)  # noqa: E731
     p("\n\n NAS Parallel Benchmarks 4.1 npbcg (gfx950 HIP) version - CG Benchmark\n")
@@ -71,7 +88,7 @@ def run(cls: str | None, n: int, nonzer: int, niter: int, shift: float, seed: fl
     m = npbcg.makea(n, nonzer, shift, seed, RCOND)
     t_init = time.perf_counter() - t0
     p(f" Initialization time = {t_init:15.3f} seconds")
-    with npbcg.Context(m, device=device) as ctx:
+    with npbcg.Context(m, device=device, use_graph=int(use_graph)) as ctx:
         ctx.fill_x(1.0)
         ctx.power(1, shift)          # untimed warm-up iteration (cg.cu:458-482)
         ctx.fill_x(1.0)              # reset x = 1 (cg.cu:485-487)
@@ -79,6 +96,12 @@ def run(cls: str | None, n: int, nonzer: int, niter: int, shift: float, seed: fl
         zeta, rnorm = ctx.power(niter, shift)
         t = time.perf_counter() - t0
         config = ctx.describe()
+        prof = None
+        if profile:
+            ctx.fill_x(1.0)
+            kernels, prof_ms, prof_zeta = ctx.profile_kernels(niter, shift)
+            prof = {"kernels": kernels, "total_ms": prof_ms,
+                    "same_zeta": bool((prof_zeta == zeta).all())}
     p("\n   iteration           ||r||                 zeta")
     for it in range(niter):
         p(f"    {it + 1:5d}       {rnorm[it]:20.14e}{zeta[it]:20.13e}")
@@ -120,10 +143,18 @@ def run(cls: str | None, n: int, nonzer: int, niter: int, shift: float, seed: fl
     p(f"    GPU device   = {config.get('device_name', 'device ' + str(device))}")
     p("\n Software:")
     p(f"    Parameters   = {json.dumps(config)}")
+    if prof is not None:
+        p(f"\n Profiled run: {niter} iterations from x = 1, kernel by kernel, "
+          f"{prof['total_ms'] / 1e3:.6f} seconds of device time; zeta "
+          f"{'identical to' if prof['same_zeta'] else 'DIFFERS from'} the timed run")
+        p(kernel_table(prof["kernels"], prof["total_ms"]))
     p("")
-    return {"class": cls or "U", "n": n, "nonzer": nonzer, "niter": niter, "shift": shift,
-            "seed": seed, "zeta": z, "zeta_error": err, "verified": verified,
-            "time_s": t, "mops": mop, "makea_s": t_init}
+    res = {"class": cls or "U", "n": n, "nonzer": nonzer, "niter": niter, "shift": shift,
+           "seed": seed, "zeta": z, "zeta_error": err, "verified": verified,
+           "time_s": t, "mops": mop, "makea_s": t_init}
+    if prof is not None:
+        res["profile"] = prof
+    return res
 
 
 def main(argv=None) -> int:
@@ -136,6 +167,11 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=float, default=NPB_SEED)
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--json", action="store_true", help="also print one JSON summary line")
+    ap.add_argument("--profile", action="store_true",
+                    help="also print per-kernel device times of a profiled repeat")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the timed loop kernel by kernel instead of replaying a "
+                         "captured graph (for tracing tools)")
     a = ap.parse_args(argv)
     if a.cls:
         n, nonzer, niter, shift, _ = CLASSES[a.cls]
@@ -146,7 +182,8 @@ def main(argv=None) -> int:
         if None in (a.n, a.nonzer, a.niter, a.shift):
             ap.error("give --class or all of --n --nonzer --niter --shift")
         n, nonzer, niter, shift, seed = a.n, a.nonzer, a.niter, a.shift, a.seed
-    res = run(a.cls, n, nonzer, niter, shift, seed, a.device)
+    res = run(a.cls, n, nonzer, niter, shift, seed, a.device, profile=a.profile,
+              use_graph=not a.no_graph)
     if a.json:
         print(json.dumps(res))
     return 0 if (res["verified"] or a.cls is None) else 1

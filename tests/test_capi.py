@@ -80,4 +80,8 @@ def test_null_context_calls_are_errors(npbcg_lib):
     assert L.npbcg_norms(None, ctypes.byref(d), ctypes.byref(d)) == npbcg_lib.ERR_INVALID
     assert L.npbcg_scale_x(None, 1.0) == npbcg_lib.ERR_INVALID
     assert L.npbcg_fill_x(None, 1.0) == npbcg_lib.ERR_INVALID
+    kt = (npbcg_lib.KernelTime * 4)()
+    nk = ctypes.c_int(0)
+    assert L.npbcg_profile_kernels(None, 1, 10.0, np.zeros(1), kt, 4, ctypes.byref(nk),
+                                   ctypes.byref(d)) == npbcg_lib.ERR_INVALID
     L.npbcg_destroy(None)

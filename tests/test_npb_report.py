@@ -39,10 +39,28 @@ def test_arguments(npbcg_lib):
         m.main(["--n", "100"])
 
 
+def test_kernel_table_layout(npbcg_lib):
+    # the columns of the reference's PROFILING table (cg.cu:566-568), plus launches
+    m = _mod()
+    text = m.kernel_table([{"name": "spmv_sell_kernel (s = A r)", "threads": 1024,
+                            "calls": 375, "ms": 3.0},
+                           {"name": "update_cg_kernel", "threads": 256, "calls": 375,
+                            "ms": 1.0}], 5.0)
+    head, row1, row2 = text.splitlines()
+    assert head.split("\t")[:4] == ["GPU Kernel", " " * 8 + "Threads Per Block",
+                                     " " * 10 + "Time in Seconds", " " * 7 + "Time in Percentage"]
+    f = row1.split("\t")
+    assert f[0].strip() == "spmv_sell_kernel (s = A r)" and len(f[0]) == 29
+    assert int(f[1]) == 1024 and float(f[2]) == 0.003 and f[3].strip() == "60.00%"
+    assert int(f[4]) == 375
+    assert row2.split("\t")[3].strip() == "20.00%"
+
+
 @pytest.mark.gpu
 def test_class_s_report_verifies(gpu):
     out = subprocess.run([sys.executable, os.path.join(ROOT, "npb-gpu_amd", "npbcg", "npb_cg.py"),
-                          "--class", "S", "--json"], capture_output=True, text=True, timeout=300)
+                          "--class", "S", "--json", "--profile"], capture_output=True, text=True,
+                         timeout=300)
     print(out.stdout[-2000:], out.stderr[-2000:])
     assert out.returncode == 0
     lines = out.stdout.splitlines()
@@ -50,3 +68,5 @@ def test_class_s_report_verifies(gpu):
     assert " Verification    =               SUCCESSFUL" in lines
     it_lines = [l for l in lines if re.fullmatch(r" {4,}\d+ +\S+e[-+]\d+ *\S+e[-+]\d+", l)]
     assert len(it_lines) == 15 and it_lines[-1].split()[0] == "15"
+    assert any("zeta identical to the timed run" in l for l in lines)
+    assert any(l.strip().startswith("spmv_") for l in lines)
