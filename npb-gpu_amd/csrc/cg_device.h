@@ -44,23 +44,38 @@ __device__ __forceinline__ double segment_sum(double v) {
     return v;
 }
 
-// One chunk of a SELL slice (entries j0 .. j0+UNR-1 of the lane's row): the
-// slice stores entry j of lane l at base + 64 j + l.  Entries past the slice
-// width re-read the slice's last entry with value 0, so every load stays
-// inside the slice, a chunk's loads can all be in flight together, and the
-// sum gains only exact zeros (a * x + 0 * y).  width must be at least 1.
+// Loads entries j0 .. j0+UNR-1 of lane `lane` of the SELL slice at `base`
+// (entry j at base + 64 j + lane) at a constant stride, so the offsets are
+// instruction immediates.  Entries at or past the slice width read the next
+// slices' storage or the zero entries every device SELL array carries after
+// its last slice (kSellTail in tiled_layout.h, 16 x 64); sell_trim clears
+// them where the chunk is used, so the loads never wait on a mask.
 template <typename T, int UNR>
 __device__ __forceinline__ void sell_chunk(const T *__restrict__ val,
-                                           const uint16_t *__restrict__ col, int base, int width,
-                                           int j0, int lane, T (&a)[UNR], int (&c)[UNR]) {
+                                           const uint16_t *__restrict__ col, int base, int j0,
+                                           int lane, T (&a)[UNR], int (&c)[UNR]) {
+    static_assert(UNR <= 16, "a chunk may read at most 16 entries past a slice");
+    const size_t at = (size_t)base + (size_t)j0 * kWave + lane;
+    const T *vp = val + at;
+    const uint16_t *cp = col + at;
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-        const int j = min(j0 + u, width - 1);
-        const size_t at = (size_t)base + (size_t)j * kWave + lane;
-        const T v = val[at];
-        c[u] = col[at];
-        a[u] = j0 + u < width ? v : T(0);
+        a[u] = vp[u * kWave];
+        c[u] = cp[u * kWave];
     }
+}
+
+// Value 0 and column 0 for the entries of a chunk at or past `valid` (the
+// slice width less the chunk's first entry; the same for the whole wavefront).
+template <typename T, int UNR>
+__device__ __forceinline__ void sell_trim(int valid, T (&a)[UNR], int (&c)[UNR]) {
+    if (valid >= UNR) return;
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+        if (u >= valid) {
+            a[u] = T(0);
+            c[u] = 0;
+        }
 }
 
 // Device-resident CG scalars of one context.  Kernels never pass values to

@@ -31,6 +31,9 @@
 
 namespace npbcg_dev {
 
+// Entries per lane in flight in the SELL SpMV stream (one chunk).
+constexpr int kSellUnr = 8;
+
 // ---------------------------------------------------------------------------
 // CSR SpMV (one group of SEG lanes per row)
 // ---------------------------------------------------------------------------
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
         cbase = slice_off[s0 + cur];
         cwidth = (slice_off[s0 + cur + 1] - cbase) / kWave;
         nrow = slice_row[(size_t)(s0 + cur) * kWave + lane];
-        if (cwidth > 0) sell_chunk<double, UNR>(val, col, cbase, cwidth, 0, lane, na, nc);
+        if (cwidth > 0) sell_chunk<double, UNR>(val, col, cbase, 0, lane, na, nc);
     }
     if constexpr (!PUPD) {
         const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
@@ -238,18 +241,19 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
                 av[u] = na[u];
                 c[u] = nc[u];
             }
+            sell_trim<double, UNR>(width - ch * UNR, av, c);
             if (ch + 1 < nch) {
-                sell_chunk<double, UNR>(val, col, base, width, (ch + 1) * UNR, lane, na, nc);
+                sell_chunk<double, UNR>(val, col, base, (ch + 1) * UNR, lane, na, nc);
             } else if (nxt >= 0) {
                 nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-                if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
+                if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, 0, lane, na, nc);
             }
 #pragma unroll
             for (int u = 0; u < UNR; u++) acc = fma(av[u], vblk[c[u]], acc);
         }
         if (nch == 0 && nxt >= 0) {
             nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-            if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
+            if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, 0, lane, na, nc);
         }
         if (row >= 0) part_b[row] = acc;
         if constexpr (MODE == 0) {
@@ -712,7 +716,7 @@ hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEve
     const int rev = MODE == 0 ? (step & 1) : 1;
     if (MODE == 0 && step > 0 && fused_p(pb)) {
         return noted(pb,
-                     launch_k(spmv_sell_kernel<BLOCK, 8, 0, true>, d.grid, BLOCK, d.lds_bytes,
+                     launch_k(spmv_sell_kernel<BLOCK, kSellUnr, 0, true>, d.grid, BLOCK, d.lds_bytes,
                               st, e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col,
                               (const double *)p_of_step(pb, step - 1), d.rows_part, pb.part.pq,
                               pb.n, d.W, d.nrange, (const double *)pb.r, p_of_step(pb, step),
@@ -724,7 +728,7 @@ hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEve
     // one-reduction form: the CG SpMV is s = A r (its dot partials are r.s)
     const double *v = MODE == 0 ? (pb.cg_form ? pb.r : p_of_step(pb, step)) : pb.z;
     return noted(pb,
-                 launch_k(spmv_sell_kernel<BLOCK, 8, MODE, false>, d.grid, BLOCK, d.lds_bytes,
+                 launch_k(spmv_sell_kernel<BLOCK, kSellUnr, MODE, false>, d.grid, BLOCK, d.lds_bytes,
                           st, e0, e1, d.wg_slice, d.slice_off, d.slice_row, d.val, d.col, v,
                           d.rows_part, pb.part.pq, pb.n, d.W, d.nrange, (const double *)nullptr,
                           (double *)nullptr, (const double *)nullptr, (const double *)nullptr,
@@ -752,7 +756,7 @@ hipError_t launch_spmv(const Fp64Problem &pb, hipStream_t st, hipEvent_t e0 = nu
 template <int BLOCK, int MODE, bool PUPD>
 hipError_t set_sell_lds_t(int bytes) {
     return hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&spmv_sell_kernel<BLOCK, 8, MODE, PUPD>),
+        reinterpret_cast<const void *>(&spmv_sell_kernel<BLOCK, kSellUnr, MODE, PUPD>),
         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -16,6 +16,9 @@
 
 namespace npbcg_dev {
 
+// Entries per lane in flight in the FP32 SELL SpMV stream (one chunk).
+constexpr int kSellUnr32 = 8;
+
 // val32 = (float) val64 (the FP32 operator).
 __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double *__restrict__ src,
                                                          float *__restrict__ dst,
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
         cbase = slice_off[s0 + cur];
         cwidth = (slice_off[s0 + cur + 1] - cbase) / kWave;
         nrow = slice_row[(size_t)(s0 + cur) * kWave + lane];
-        if (cwidth > 0) sell_chunk<float, UNR>(val, col, cbase, cwidth, 0, lane, na, nc);
+        if (cwidth > 0) sell_chunk<float, UNR>(val, col, cbase, 0, lane, na, nc);
     }
     if constexpr (PUPD) {
         const float2 *v2 = reinterpret_cast<const float2 *>(v + c0);
@@ -142,18 +145,19 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
                 av[u] = na[u];
                 c[u] = nc[u];
             }
+            sell_trim<float, UNR>(width - ch * UNR, av, c);
             if (ch + 1 < nch) {
-                sell_chunk<float, UNR>(val, col, base, width, (ch + 1) * UNR, lane, na, nc);
+                sell_chunk<float, UNR>(val, col, base, (ch + 1) * UNR, lane, na, nc);
             } else if (nxt >= 0) {
                 nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-                if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
+                if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, 0, lane, na, nc);
             }
 #pragma unroll
             for (int u = 0; u < UNR; u++) acc = fmaf(av[u], vblk[c[u]], acc);
         }
         if (nch == 0 && nxt >= 0) {
             nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-            if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
+            if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, 0, lane, na, nc);
         }
         if (row >= 0) part_b[row] = acc;
         const double d = segment_sum<64>((double)pr * (double)acc);
@@ -364,10 +368,10 @@ __global__ __launch_bounds__(kBlock) void jacobi_kernel(const int *__restrict__ 
 
 hipError_t prepare_mixed_kernels(int lds_bytes) {
     hipError_t last = hipSuccess;
-    for (const void *f : {reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, 8, false>),
-                          reinterpret_cast<const void *>(&spmv_sell32_kernel<512, 8, false>),
-                          reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, 8, true>),
-                          reinterpret_cast<const void *>(&spmv_sell32_kernel<512, 8, true>)}) {
+    for (const void *f : {reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, kSellUnr32, false>),
+                          reinterpret_cast<const void *>(&spmv_sell32_kernel<512, kSellUnr32, false>),
+                          reinterpret_cast<const void *>(&spmv_sell32_kernel<1024, kSellUnr32, true>),
+                          reinterpret_cast<const void *>(&spmv_sell32_kernel<512, kSellUnr32, true>)}) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
         if (e != hipSuccess) last = e;
     }
@@ -380,11 +384,11 @@ static hipError_t launch_spmv32(const Fp64Problem &pb, const MixedProblem &mp, c
                                 int rev, hipStream_t st) {
     const SellDev &sd = pb.sell;
     if (sd.block == 512)
-        spmv_sell32_kernel<512, 8, PUPD><<<sd.grid, 512, mp.lds_bytes, st>>>(
+        spmv_sell32_kernel<512, kSellUnr32, PUPD><<<sd.grid, 512, mp.lds_bytes, st>>>(
             sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
             pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev);
     else
-        spmv_sell32_kernel<1024, 8, PUPD><<<sd.grid, 1024, mp.lds_bytes, st>>>(
+        spmv_sell32_kernel<1024, kSellUnr32, PUPD><<<sd.grid, 1024, mp.lds_bytes, st>>>(
             sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
             pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev);
     return hipGetLastError();
