@@ -44,6 +44,25 @@ __device__ __forceinline__ double segment_sum(double v) {
     return v;
 }
 
+// Copies count elements src[0..count) to LDS dst[0..count) with BLOCK
+// threads.  Every thread first issues its loads for the first FILL * BLOCK
+// elements (FILL in flight at once, held in registers), then writes them: the
+// copy costs one memory round trip instead of one per few elements.  Indices
+// past the end are clamped to the last element on both sides (those lanes
+// write its own value again), so neither loop branches and the compiler
+// cannot sink a load next to its store.  Elements past FILL * BLOCK (none for
+// the block sizes used) follow in a plain loop.
+template <int BLOCK, int FILL, typename V>
+__device__ __forceinline__ void stage_block(const V *__restrict__ src, V *dst, int count) {
+    if (count <= 0) return;
+    V t[FILL];
+#pragma unroll
+    for (int k = 0; k < FILL; k++) t[k] = src[min((int)threadIdx.x + k * BLOCK, count - 1)];
+#pragma unroll
+    for (int k = 0; k < FILL; k++) dst[min((int)threadIdx.x + k * BLOCK, count - 1)] = t[k];
+    for (int i = threadIdx.x + FILL * BLOCK; i < count; i += BLOCK) dst[i] = src[i];
+}
+
 // Loads entries j0 .. j0+UNR-1 of lane `lane` of the SELL slice at `base`
 // (entry j at base + 64 j + lane) at a constant stride, so the offsets are
 // instruction immediates.  Entries at or past the slice width read the next

@@ -212,9 +212,9 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
         if (cwidth > 0) sell_chunk<double, UNR>(val, col, cbase, 0, lane, na, nc);
     }
     if constexpr (!PUPD) {
-        const double2 *v2 = reinterpret_cast<const double2 *>(v + c0);
-#pragma unroll 4
-        for (int i = threadIdx.x; i < wb / 2; i += BLOCK) lds_v2[i] = v2[i];
+        // a block of up to 20,480 doubles: 10 (or 20) 16-byte loads per thread
+        stage_block<BLOCK, 10240 / BLOCK>(reinterpret_cast<const double2 *>(v + c0), lds_v2,
+                                          wb / 2);
         if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
     }
     __syncthreads();

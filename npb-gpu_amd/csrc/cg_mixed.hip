@@ -115,11 +115,10 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
             if (store) p_new[c0 + wb - 1] = nv;
         }
     } else {
-        // c0 is even (W even); stage pairs, then a possible odd tail.
-        const float2 *v2 = reinterpret_cast<const float2 *>(v + c0);
-        float2 *l2 = reinterpret_cast<float2 *>(vblk);
-#pragma unroll 4
-        for (int i = threadIdx.x; i < wb / 2; i += BLOCK) l2[i] = v2[i];
+        // c0 is even (W even); stage pairs (up to 10 in flight per thread),
+        // then a possible odd tail.
+        stage_block<BLOCK, 10240 / BLOCK>(reinterpret_cast<const float2 *>(v + c0),
+                                          reinterpret_cast<float2 *>(vblk), wb / 2);
         if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
     }
     __syncthreads();
