@@ -693,17 +693,11 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
                        b->stream));
     TRY(hipMalloc(&b->d_sysout, sizeof(double) * nsys * b->nr));
     if (b->sell_ok) {
-        // kSellTail zero entries after the last slice (chunk loads read past it)
         const size_t stored = scol.size();
-        const size_t ent = stored + npbcg_detail::kSellTail;
-        const size_t vsize = o.mode == NPBCG_SOLVE_MIXED ? sizeof(float) : sizeof(double);
-        const size_t vbytes = vsize * stored;
-        TRY(hipMalloc(&b->d_sell_val, vsize * ent));
-        TRY(hipMalloc(&b->d_sell_col, sizeof(uint16_t) * ent));
-        TRY(hipMemsetAsync(static_cast<char *>(b->d_sell_val) + vbytes, 0,
-                           vsize * npbcg_detail::kSellTail, b->stream));
-        TRY(hipMemsetAsync(b->d_sell_col + stored, 0, sizeof(uint16_t) * npbcg_detail::kSellTail,
-                           b->stream));
+        const size_t vbytes = o.mode == NPBCG_SOLVE_MIXED ? sizeof(float) * stored
+                                                          : sizeof(double) * stored;
+        TRY(hipMalloc(&b->d_sell_val, std::max<size_t>(vbytes, 8)));
+        TRY(hipMalloc(&b->d_sell_col, std::max<size_t>(sizeof(uint16_t) * stored, 8)));
         TRY(hipMalloc(&b->d_sell_off, sizeof(int) * soff.size()));
         TRY(hipMalloc(&b->d_sell_row, sizeof(int) * std::max<size_t>(srow.size(), 1)));
         const void *vsrc = o.mode == NPBCG_SOLVE_MIXED ? (const void *)sval32.data()
@@ -719,13 +713,8 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     if (b->res_ok) {
         if (b->res_owns) {
             const size_t stored = rcol.size();
-            const size_t ent = stored + npbcg_detail::kSellTail;
-            TRY(hipMalloc(&b->d_res_val, sizeof(double) * ent));
-            TRY(hipMalloc(&b->d_res_col, sizeof(uint16_t) * ent));
-            TRY(hipMemsetAsync(b->d_res_val + stored, 0, sizeof(double) * npbcg_detail::kSellTail,
-                               b->stream));
-            TRY(hipMemsetAsync(b->d_res_col + stored, 0,
-                               sizeof(uint16_t) * npbcg_detail::kSellTail, b->stream));
+            TRY(hipMalloc(&b->d_res_val, std::max<size_t>(sizeof(double) * stored, 8)));
+            TRY(hipMalloc(&b->d_res_col, std::max<size_t>(sizeof(uint16_t) * stored, 8)));
             TRY(hipMalloc(&b->d_res_off, sizeof(int) * roff.size()));
             TRY(hipMalloc(&b->d_res_row, sizeof(int) * std::max<size_t>(rrow.size(), 1)));
             TRY(hipMalloc(&b->d_res_rows_part,

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
         cbase = off[cur];
         cwidth = (off[cur + 1] - cbase) / kWave;
         nrow = lane_row[(size_t)cur * kWave + lane];
-        if (cwidth > 0) sell_chunk<T, UNR>(sval, col, cbase, 0, lane, na, nc);
+        if (cwidth > 0) sell_chunk<T, UNR>(sval, col, cbase, cwidth, 0, lane, na, nc);
     }
     {
         const T *src = p + (size_t)(srow + c0) * NR;
@@ -373,10 +373,10 @@ __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
             }
             sell_trim<T, UNR>(width - ch * UNR, a, c);
             if (ch + 1 < nch) {
-                sell_chunk<T, UNR>(sval, col, base, (ch + 1) * UNR, lane, na, nc);
+                sell_chunk<T, UNR>(sval, col, base, width, (ch + 1) * UNR, lane, na, nc);
             } else if (nxt >= 0) {
                 nrow = lane_row[(size_t)nxt * kWave + lane];
-                if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, 0, lane, na, nc);
+                if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
             }
 #pragma unroll
             for (int u = 0; u < UNR; u++) {
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
         }
         if (nch == 0 && nxt >= 0) {
             nrow = lane_row[(size_t)nxt * kWave + lane];
-            if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, 0, lane, na, nc);
+            if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
         }
         if (row >= 0) {
             VecT<T, NR> out;

@@ -517,13 +517,9 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
         const size_t nwg1 = sm.wg_slice.size(), nsl1 = sm.slice_off.size();
         const size_t idx = nwg1 + nsl1 + sm.slice_row.size();
         CREATE_HIP(hipMalloc(&c->d_sell_idx, sizeof(int) * idx));
-        const size_t ent = sm.val.size() + npbcg_detail::kSellTail;
-        CREATE_HIP(hipMalloc(&c->d_sell_val, sizeof(double) * ent));
-        CREATE_HIP(hipMalloc(&c->d_sell_col, sizeof(uint16_t) * ent));
-        CREATE_HIP(hipMemset(c->d_sell_val + sm.val.size(), 0,
-                             sizeof(double) * npbcg_detail::kSellTail));
-        CREATE_HIP(hipMemset(c->d_sell_col + sm.val.size(), 0,
-                             sizeof(uint16_t) * npbcg_detail::kSellTail));
+        CREATE_HIP(hipMalloc(&c->d_sell_val, sizeof(double) * std::max<size_t>(1, sm.val.size())));
+        CREATE_HIP(hipMalloc(&c->d_sell_col,
+                             sizeof(uint16_t) * std::max<size_t>(1, sm.col.size())));
         CREATE_HIP(hipMalloc(&c->d_rows_part, sizeof(double) * (size_t)sm.nblk * n));
         CREATE_HIP(hipMemcpy(c->d_sell_idx, sm.wg_slice.data(), sizeof(int) * nwg1,
                              hipMemcpyHostToDevice));
@@ -1093,11 +1089,9 @@ namespace {
 int ensure_mixed(npbcg_ctx *c) {
     if (c->d_val32) return NPBCG_OK;
     const npbcg_dev::SellDev &sd = c->pb.sell;
-    const size_t stored = (size_t)c->sell_stored;
+    const size_t stored = (size_t)std::max<long long>(1, c->sell_stored);
     const size_t vb = (size_t)c->pb.vec_blocks;
-    NPBCG_HIP(hipMalloc(&c->d_val32, sizeof(float) * (stored + npbcg_detail::kSellTail)));
-    NPBCG_HIP(hipMemsetAsync(c->d_val32 + stored, 0, sizeof(float) * npbcg_detail::kSellTail,
-                             c->stream));
+    NPBCG_HIP(hipMalloc(&c->d_val32, sizeof(float) * stored));
     NPBCG_HIP(hipMalloc(&c->d_vec32, sizeof(float) * 5 * c->vstride));
     NPBCG_HIP(hipMalloc(&c->d_rows32, sizeof(float) * (size_t)sd.nblk * c->n));
     NPBCG_HIP(hipMalloc(&c->d_part32, sizeof(double) * (5 * vb + (size_t)sd.grid)));

@@ -64,23 +64,32 @@ __device__ __forceinline__ void stage_block(const V *__restrict__ src, V *dst, i
 }
 
 // Loads entries j0 .. j0+UNR-1 of lane `lane` of the SELL slice at `base`
-// (entry j at base + 64 j + lane) at a constant stride, so the offsets are
-// instruction immediates.  Entries at or past the slice width read the next
-// slices' storage or the zero entries every device SELL array carries after
-// its last slice (kSellTail in tiled_layout.h, 16 x 64); sell_trim clears
-// them where the chunk is used, so the loads never wait on a mask.
+// (entry j at base + 64 j + lane; j0 < width).  A full chunk loads at a
+// constant stride, so the offsets are instruction immediates; the slice's
+// last chunk (a wave-uniform case) clamps the index to width - 1, so no load
+// leaves the slice.  The entries at or past the width are cleared by
+// sell_trim where the chunk is used, so the loads never wait on a mask.
 template <typename T, int UNR>
 __device__ __forceinline__ void sell_chunk(const T *__restrict__ val,
-                                           const uint16_t *__restrict__ col, int base, int j0,
-                                           int lane, T (&a)[UNR], int (&c)[UNR]) {
-    static_assert(UNR <= 16, "a chunk may read at most 16 entries past a slice");
+                                           const uint16_t *__restrict__ col, int base, int width,
+                                           int j0, int lane, T (&a)[UNR], int (&c)[UNR]) {
     const size_t at = (size_t)base + (size_t)j0 * kWave + lane;
     const T *vp = val + at;
     const uint16_t *cp = col + at;
+    if (j0 + UNR <= width) {
 #pragma unroll
-    for (int u = 0; u < UNR; u++) {
-        a[u] = vp[u * kWave];
-        c[u] = cp[u * kWave];
+        for (int u = 0; u < UNR; u++) {
+            a[u] = vp[u * kWave];
+            c[u] = cp[u * kWave];
+        }
+    } else {
+        const int last = width - 1 - j0;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int k = min(u, last) * kWave;
+            a[u] = vp[k];
+            c[u] = cp[k];
+        }
     }
 }
 

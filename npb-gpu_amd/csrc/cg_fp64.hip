@@ -209,7 +209,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
         cbase = slice_off[s0 + cur];
         cwidth = (slice_off[s0 + cur + 1] - cbase) / kWave;
         nrow = slice_row[(size_t)(s0 + cur) * kWave + lane];
-        if (cwidth > 0) sell_chunk<double, UNR>(val, col, cbase, 0, lane, na, nc);
+        if (cwidth > 0) sell_chunk<double, UNR>(val, col, cbase, cwidth, 0, lane, na, nc);
     }
     if constexpr (!PUPD) {
         // a block of up to 20,480 doubles: 10 (or 20) 16-byte loads per thread
@@ -243,17 +243,17 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
             }
             sell_trim<double, UNR>(width - ch * UNR, av, c);
             if (ch + 1 < nch) {
-                sell_chunk<double, UNR>(val, col, base, (ch + 1) * UNR, lane, na, nc);
+                sell_chunk<double, UNR>(val, col, base, width, (ch + 1) * UNR, lane, na, nc);
             } else if (nxt >= 0) {
                 nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-                if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, 0, lane, na, nc);
+                if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
             }
 #pragma unroll
             for (int u = 0; u < UNR; u++) acc = fma(av[u], vblk[c[u]], acc);
         }
         if (nch == 0 && nxt >= 0) {
             nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-            if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, 0, lane, na, nc);
+            if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
         }
         if (row >= 0) part_b[row] = acc;
         if constexpr (MODE == 0) {

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
         cbase = slice_off[s0 + cur];
         cwidth = (slice_off[s0 + cur + 1] - cbase) / kWave;
         nrow = slice_row[(size_t)(s0 + cur) * kWave + lane];
-        if (cwidth > 0) sell_chunk<float, UNR>(val, col, cbase, 0, lane, na, nc);
+        if (cwidth > 0) sell_chunk<float, UNR>(val, col, cbase, cwidth, 0, lane, na, nc);
     }
     if constexpr (PUPD) {
         const float2 *v2 = reinterpret_cast<const float2 *>(v + c0);
@@ -146,17 +146,17 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
             }
             sell_trim<float, UNR>(width - ch * UNR, av, c);
             if (ch + 1 < nch) {
-                sell_chunk<float, UNR>(val, col, base, (ch + 1) * UNR, lane, na, nc);
+                sell_chunk<float, UNR>(val, col, base, width, (ch + 1) * UNR, lane, na, nc);
             } else if (nxt >= 0) {
                 nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-                if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, 0, lane, na, nc);
+                if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
             }
 #pragma unroll
             for (int u = 0; u < UNR; u++) acc = fmaf(av[u], vblk[c[u]], acc);
         }
         if (nch == 0 && nxt >= 0) {
             nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
-            if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, 0, lane, na, nc);
+            if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
         }
         if (row >= 0) part_b[row] = acc;
         const double d = segment_sum<64>((double)pr * (double)acc);

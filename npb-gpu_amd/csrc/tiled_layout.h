@@ -23,10 +23,6 @@
 namespace npbcg_detail {
 
 constexpr int kSliceRows = 64;
-// Extra entries every device copy of a SELL value or column array holds
-// after its last slice (zeros): the SpMV kernels load whole chunks of up to
-// 16 entries per lane and clear the ones past the slice width after loading.
-constexpr int kSellTail = 16 * kSliceRows;
 
 template <class V>
 struct SellBlocks {
