@@ -73,6 +73,14 @@ template <typename T, int UNR>
 __device__ __forceinline__ void sell_chunk(const T *__restrict__ val,
                                            const uint16_t *__restrict__ col, int base, int width,
                                            int j0, int lane, T (&a)[UNR], int (&c)[UNR]) {
+    if constexpr (UNR > 8) {
+        // base, width and j0 are the same for the whole wavefront; saying so
+        // keeps the clamped path's offsets scalar (without it a 12-entry FP64
+        // chunk exceeds 128 VGPRs and spills; at 8 entries it is slower)
+        base = __builtin_amdgcn_readfirstlane(base);
+        width = __builtin_amdgcn_readfirstlane(width);
+        j0 = __builtin_amdgcn_readfirstlane(j0);
+    }
     const size_t at = (size_t)base + (size_t)j0 * kWave + lane;
     const T *vp = val + at;
     const uint16_t *cp = col + at;

@@ -31,8 +31,9 @@
 
 namespace npbcg_dev {
 
-// Entries per lane in flight in the SELL SpMV stream (one chunk).
-constexpr int kSellUnr = 8;
+// Entries per lane in flight in the SELL SpMV stream (one chunk): 12 x 10 B
+// per lane, the most that fits 128 VGPRs with the prefetched chunk.
+constexpr int kSellUnr = 12;
 
 // ---------------------------------------------------------------------------
 // CSR SpMV (one group of SEG lanes per row)
