@@ -136,3 +136,34 @@ def test_batch_sell_many_blocks_and_windows(gpu):
                 b = rhs[s][:, k]
                 ox, _, _ = oracle.cg_solve(om, b, rtol=RTOL, maxit=5000)
                 assert np.max(np.abs(xs[s][:, k] - ox)) <= 1e-8 * np.max(np.abs(ox))
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_batch_full_workload(gpu, mode):
+    # The full batched workload: 64 systems, n ~ U{2000..40000}, nonzer ~
+    # U{7..13} (numpy PCG64 seeded s), shift 20, matrix seeds 1..64, four
+    # right-hand sides uniform(-1, 1) each (PCG64 seeded 1000 + s), default
+    # launch shape.  Every solve must meet 1e-10; the FP64 residual of every
+    # eighth system is recomputed on the host.
+    params = []
+    for s in range(1, 65):
+        rng = np.random.default_rng(s)
+        n = int(rng.integers(2000, 40001))
+        nz = int(rng.integers(7, 14))
+        params.append((n, nz, 20.0, float(s)))
+    mats = gpu.make_many(params)
+    rhs = [np.random.default_rng(1000 + s).uniform(-1.0, 1.0, (m.n, 4))
+           for s, m in enumerate(mats, start=1)]
+    with gpu.Batch(mats, nrhs=4, mode=mode) as bt:
+        bt.set_rhs(rhs)
+        st, steps, rel = bt.solve(RTOL, 3000)
+        xs = bt.solution()
+    print(f"mode {mode}: {st.as_dict()}; max steps {int(steps.max())}")
+    assert st.solves == 256 and st.converged == 256 and st.max_relres <= RTOL
+    assert np.all(rel <= RTOL)
+    for s in range(0, 64, 8):
+        om = _ocsr(mats[s])
+        for k in range(4):
+            b = rhs[s][:, k]
+            host = np.linalg.norm(b - oracle.spmv(om, xs[s][:, k])) / np.linalg.norm(b)
+            assert host <= RTOL and abs(host - rel[s, k]) <= 1e-3 * RTOL + 0.05 * host
