@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define NPBCG_ABI_VERSION 2
+#define NPBCG_ABI_VERSION 3
 
 typedef enum npbcg_status {
     NPBCG_OK = 0,
@@ -30,11 +30,21 @@ typedef enum npbcg_status {
     NPBCG_ERR_NOCONV = -8     /* a solve stopped before reaching its tolerance */
 } npbcg_status;
 
-/* Inner-loop engines for the FP64 conj_grad. */
+/* Inner-loop engines for the FP64 conj_grad (npbcg_conj_grad,
+ * npbcg_conj_grad_device, npbcg_power).  Both run the one-reduction CG step
+ * (npbcg_opts.cg_form) and keep every scalar on the GPU; the reference issues
+ * 6 launches and 2 blocking copies per CG step (cg.cu:814-837). */
 enum {
-    NPBCG_ENGINE_AUTO = 0,
-    NPBCG_ENGINE_KERNELS = 1,     /* fused per-phase kernels, scalars stay on the GPU */
-    NPBCG_ENGINE_PERSISTENT = 2   /* one cooperative kernel runs all cgitmax iterations */
+    NPBCG_ENGINE_AUTO = 0,        /* KERNELS */
+    NPBCG_ENGINE_KERNELS = 1,     /* fused per-phase kernels (two a CG step), replayed
+                                     from a captured HIP graph per outer iteration */
+    NPBCG_ENGINE_PERSISTENT = 2   /* one cooperative kernel runs whole outer iterations
+                                     (all cgitmax steps, the residual, zeta and the
+                                     normalisation; npbcg_power: every iteration in one
+                                     launch) with grid barriers between its phases.
+                                     Needs the LDS-staged layout with every workgroup
+                                     resident: npbcg_create returns NPBCG_ERR_INVALID
+                                     otherwise.  Solves ignore the engine. */
 };
 
 /* Vectors held by a context (npbcg_get_vector / npbcg_set_vector). */
@@ -62,8 +72,21 @@ typedef struct npbcg_opts {
                          one-reduction form: s = A r, then one update forming p, q = A p,
                          p.q = r.s - beta rho / alpha_prev, z and r (two launches a step).
                          Solves always use the reference's step order. */
-    int reserved[6];
+    int matrix_check; /* npbcg_conj_grad / npbcg_compat_conj_grad: how the host arrays
+                         are checked against the context's matrix (NPBCG_CHECK_*) */
+    int reserved[5];
 } npbcg_opts;
+
+/* Check of the host arrays handed to npbcg_conj_grad (npbcg_opts.matrix_check). */
+enum {
+    NPBCG_CHECK_FULL = 0,    /* rowstr compared and colidx/a digested on every call: an
+                                in-place edit is seen (the serial conj_grad reads its
+                                arrays on every call, cg.cu:703-709) */
+    NPBCG_CHECK_SAMPLED = 1  /* rowstr compared, colidx/a by pointer or 62 sampled
+                                entries: the reference GPU path's contract (the matrix is
+                                uploaded once by setup_gpu, cg.cu:1390-1603, and the
+                                caller does not edit it in place) */
+};
 
 /* Timing of the last npbcg_power / npbcg_conj_grad_device call, from HIP
  * events recorded on the context's stream. */
@@ -119,8 +142,9 @@ void npbcg_destroy(npbcg_ctx *ctx);
  * steps), p, q, r (r = A z on exit) and *rnorm = ||x - A z|| are written,
  * n+1 entries each; index n gets q=z=0, r=p=x[n] as cg.cu:671-676 leaves it.
  * colidx/rowstr/a must describe the matrix the context was created with:
- * rowstr is compared in full and colidx/a by pointer or sampled content;
- * a mismatch returns NPBCG_ERR_MISMATCH. */
+ * rowstr is compared in full and colidx/a digested in full (or, with
+ * NPBCG_CHECK_SAMPLED, by pointer or sampled content); a mismatch, an
+ * in-place edit included, returns NPBCG_ERR_MISMATCH. */
 int npbcg_conj_grad(npbcg_ctx *ctx, int colidx[], int rowstr[], double x[],
                     double z[], double a[], double p[], double q[], double r[],
                     double *rnorm);

@@ -10,6 +10,7 @@
 #include <limits>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cg_device.h"
@@ -76,6 +77,9 @@ struct npbcg_ctx {
     int hist_cap = 0;
     Fp64Problem pb{};
 
+    // persistent engine: partials, barrier words, argument block (lazy)
+    npbcg_dev::PersistentState ps{};
+
     // Captured outer iterations: [0] first (x used as is), [1] later ones
     // (x = z/||z|| of the previous iteration applied first).
     hipGraphExec_t outer_graph[2] = {nullptr, nullptr};
@@ -91,6 +95,7 @@ struct npbcg_ctx {
     std::vector<int> h_rowstr;  // for the drop-in argument check
     const int *h_colidx_ptr = nullptr;
     const double *h_a_ptr = nullptr;
+    uint64_t matrix_digest = 0;  // of colidx and a at create (full check)
     std::vector<int> colidx_sample;
     std::vector<double> a_sample;
 
@@ -131,9 +136,14 @@ struct npbcg_ctx {
         int loaded = -1;         // preconditioner kind whose values op holds
     } g;
 
-    // step budgets learned from the last converged solve of each mode (0: the
-    // defaults): FP64 steps of the first batch; mixed inner steps of sweeps 1, 2
-    int budget[2][2] = {{0, 0}, {0, 0}};
+    // the preconditioner the solve views point at (select_precond); differs
+    // from `precond` while a learned solve is redone with Jacobi
+    int active_precond = NPBCG_PRECOND_NONE;
+
+    // step budgets learned from the last converged solve of each (mode,
+    // preconditioner in effect) (0: the defaults): FP64 steps of the first
+    // batch; mixed inner steps of sweeps 1, 2
+    int budget[2][4][2] = {};
 
     // captured solve sequences, one per mode; replayed while the key matches
     struct SolveGraph {
@@ -239,6 +249,81 @@ int check_csr(int n, const int *rowstr, const int *colidx, long long *nnz_out,
     return NPBCG_OK;
 }
 
+// Digest of `bytes` bytes at p (whole 8-byte words, then the tail): four
+// independent lanes h = ((h ^ w) * K) ^ (h >> 29) over the words i = lane
+// (mod 4), folded in lane order.  Each step is a bijection of h for a fixed
+// word and an injection of the word for a fixed h, so a change of any single
+// word always changes the digest.
+uint64_t digest_bytes(const unsigned char *p, size_t bytes, uint64_t seed) {
+    constexpr uint64_t K = 0x9E3779B97F4A7C15ull;
+    uint64_t h[4] = {seed ^ 0x243F6A8885A308D3ull, seed ^ 0x13198A2E03707344ull,
+                     seed ^ 0xA4093822299F31D0ull, seed ^ 0x082EFA98EC4E6C89ull};
+    auto mix = [](uint64_t x, uint64_t w) {
+        x = (x ^ w) * K;
+        return x ^ (x >> 29);
+    };
+    const size_t words = bytes / 8;
+    size_t i = 0;
+    for (; i + 4 <= words; i += 4) {
+        uint64_t w[4];
+        std::memcpy(w, p + 8 * i, 32);
+        h[0] = mix(h[0], w[0]);
+        h[1] = mix(h[1], w[1]);
+        h[2] = mix(h[2], w[2]);
+        h[3] = mix(h[3], w[3]);
+    }
+    for (; i < words; i++) {
+        uint64_t w;
+        std::memcpy(&w, p + 8 * i, 8);
+        h[i & 3] = mix(h[i & 3], w);
+    }
+    uint64_t tail = 0;
+    std::memcpy(&tail, p + 8 * words, bytes - 8 * words);
+    uint64_t out = mix(mix(0, bytes), tail);
+    for (int k = 0; k < 4; k++) out = mix(out, h[k]);
+    return out;
+}
+
+// Digest of colidx[0..nnz) and a[0..nnz): the arrays are cut into up to 16
+// equal chunks hashed by parallel host threads (one thread below 4 MB); the
+// chunk digests are folded in chunk order, so the value does not depend on
+// the thread count.
+uint64_t matrix_digest(const int *colidx, const double *a, long long nnz) {
+    if (nnz <= 0) return 0;
+    const unsigned char *src[2] = {reinterpret_cast<const unsigned char *>(colidx),
+                                   reinterpret_cast<const unsigned char *>(a)};
+    const size_t len[2] = {sizeof(int) * (size_t)nnz, sizeof(double) * (size_t)nnz};
+    constexpr int kChunks = 16;
+    uint64_t part[2][kChunks] = {};
+    auto chunk = [&](int arr, int k) {
+        const size_t per = ((len[arr] + kChunks - 1) / kChunks + 7) & ~(size_t)7;
+        const size_t b = std::min(len[arr], per * (size_t)k);
+        const size_t e = std::min(len[arr], b + per);
+        part[arr][k] = digest_bytes(src[arr] + b, e - b, (uint64_t)(arr * kChunks + k));
+    };
+    const size_t total = len[0] + len[1];
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nthreads = total < ((size_t)4 << 20) ? 1 : (int)std::min<unsigned>(16, hw);
+    if (nthreads == 1) {
+        for (int arr = 0; arr < 2; arr++)
+            for (int k = 0; k < kChunks; k++) chunk(arr, k);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&, t]() {
+                for (int w = t; w < 2 * kChunks; w += nthreads) chunk(w / kChunks, w % kChunks);
+            });
+        for (auto &th : pool) th.join();
+    }
+    uint64_t out = 0;
+    for (int arr = 0; arr < 2; arr++)
+        for (int k = 0; k < kChunks; k++) {
+            out = (out ^ part[arr][k]) * 0x9E3779B97F4A7C15ull;
+            out ^= out >> 31;
+        }
+    return out;
+}
+
 // Samples used to check that the arrays handed to npbcg_conj_grad are the
 // ones the context holds (rowstr is compared in full).
 void take_samples(npbcg_ctx *c, const int *colidx, const double *a) {
@@ -255,10 +340,19 @@ void take_samples(npbcg_ctx *c, const int *colidx, const double *a) {
     }
 }
 
+// Whether the host arrays describe the matrix the context holds.  rowstr is
+// compared in full.  With the full check (the default) colidx and a are
+// digested on every call, so an in-place edit between calls is seen, as the
+// reference's serial conj_grad reads its arrays on every call (cg.cu:703-709);
+// with the sampled check the caller keeps the reference GPU path's contract
+// (the matrix is uploaded once by setup_gpu, cg.cu:1390-1603, and never
+// edited) and only the pointers and 62 sampled entries are compared.
 bool same_matrix(const npbcg_ctx *c, const int *rowstr, const int *colidx,
                  const double *a) {
     if (std::memcmp(rowstr, c->h_rowstr.data(), sizeof(int) * (c->n + 1)) != 0)
         return false;
+    if (c->opts.matrix_check == NPBCG_CHECK_FULL)
+        return matrix_digest(colidx, a, c->nnz) == c->matrix_digest;
     if (colidx == c->h_colidx_ptr && a == c->h_a_ptr) return true;
     const long long step = std::max<long long>(1, c->nnz / 61);
     size_t i = 0;
@@ -324,6 +418,29 @@ int ensure_probes(npbcg_ctx *c) {
         c->probe_start.push_back(a);
         c->probe_stop.push_back(b);
     }
+    return NPBCG_OK;
+}
+
+// niter outer iterations through the persistent kernel, in launches short
+// enough that the barrier counter cannot wrap, then the abort word: nonzero
+// means a grid barrier timed out and the results are not valid.
+int run_persistent(npbcg_ctx *c, int niter, int scale_first, int record, int final_scale) {
+    const long long per_iter = npbcg_dev::persistent_barriers(c->cgitmax) * c->pb.sell.grid;
+    const int chunk = (int)std::max<long long>(1, std::min<long long>(niter, (1LL << 31) / per_iter));
+    for (int done = 0; done < niter; done += chunk) {
+        const int k = std::min(chunk, niter - done);
+        NPBCG_HIP(npbcg_dev::launch_persistent(c->pb, c->ps, k, (done > 0 || scale_first) ? 1 : 0,
+                                               record, (final_scale && done + k == niter) ? 1 : 0,
+                                               c->stream));
+    }
+    unsigned words[2] = {0, 0};
+    NPBCG_HIP(hipMemcpyAsync(c->h_stage, c->ps.bar, sizeof(words), hipMemcpyDeviceToHost,
+                             c->stream));
+    NPBCG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(words, c->h_stage, sizeof(words));
+    if (words[1] != 0)
+        return set_error(NPBCG_ERR_DEVICE,
+                         "persistent conj_grad: a grid barrier timed out (results invalid)");
     return NPBCG_OK;
 }
 
@@ -398,7 +515,14 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     if (opts) c->opts = *opts;
     c->device = (c->opts.device >= 0 && c->opts.device < ndev) ? c->opts.device : 0;
     c->cgitmax = c->opts.cgitmax > 0 ? c->opts.cgitmax : 25;
-    c->engine = NPBCG_ENGINE_KERNELS;
+    if (c->opts.engine < NPBCG_ENGINE_AUTO || c->opts.engine > NPBCG_ENGINE_PERSISTENT) {
+        delete c;
+        return set_error(NPBCG_ERR_INVALID, "npbcg_create: unknown engine");
+    }
+    // AUTO takes the graph-replayed kernels: measured faster than the
+    // persistent kernel on this path (see npbcg.h)
+    c->engine = c->opts.engine == NPBCG_ENGINE_PERSISTENT ? NPBCG_ENGINE_PERSISTENT
+                                                          : NPBCG_ENGINE_KERNELS;
     c->n = n;
     c->nnz = nnz;
     c->cols_sorted = sorted;
@@ -567,6 +691,30 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
                      c->opts.cg_form != 1) ? 1 : 0;
     c->pb.sell = sd;
     refresh_solve_view(c);
+    if (c->engine == NPBCG_ENGINE_PERSISTENT) {
+        // one cooperative launch per call: every workgroup of the SELL grid must
+        // be resident at once, or the launch is refused here rather than later
+        if (variant != npbcg_dev::kSpmvTiled)
+            return fail(set_error(NPBCG_ERR_INVALID,
+                                  "npbcg_create: the persistent engine needs the LDS-staged "
+                                  "layout (spmv_variant 5 with sorted columns)"));
+        int coop = 0, per_cu = 0, clock_khz = 0;
+        CREATE_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device));
+        CREATE_HIP(npbcg_dev::persistent_capacity(c->pb, &per_cu));
+        if (!coop || (long long)per_cu * cus < sd.grid)
+            return fail(set_error(NPBCG_ERR_INVALID,
+                                  "npbcg_create: the persistent engine's " +
+                                      std::to_string(sd.grid) +
+                                      " workgroups cannot all be resident (" +
+                                      std::to_string(per_cu) + " per CU)"));
+        CREATE_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c->device));
+        CREATE_HIP(hipMalloc(&c->ps.part, sizeof(double) * npbcg_dev::persistent_partials(c->pb)));
+        CREATE_HIP(hipMalloc(&c->ps.bar, 2 * sizeof(unsigned)));
+        CREATE_HIP(hipMalloc(&c->ps.args, npbcg_dev::persistent_args_bytes()));
+        // a barrier that waits 2 s has lost a workgroup: the kernel gives up
+        c->ps.spin_limit = (long long)std::max(clock_khz, 1000) * 2000;
+        c->ps.cgitmax = c->cgitmax;
+    }
     CREATE_HIP(npbcg_dev::launch_fill(c->pb.x, n + 1, 1.0, vec_blocks, c->stream));
     CREATE_HIP(hipStreamSynchronize(c->stream));
 #undef CREATE_HIP
@@ -575,6 +723,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     c->h_colidx_ptr = colidx;
     c->h_a_ptr = a;
     take_samples(c, colidx, a);
+    if (c->opts.matrix_check == NPBCG_CHECK_FULL) c->matrix_digest = matrix_digest(colidx, a, nnz);
     c->timing.n = n;
     c->timing.nnz = nnz;
     c->timing.engine = c->engine;
@@ -622,6 +771,9 @@ void npbcg_destroy(npbcg_ctx *c) {
     (void)hipFree(c->d_part);
     (void)hipFree(c->d_sc);
     (void)hipFree(c->d_hist);
+    (void)hipFree(c->ps.part);
+    (void)hipFree(c->ps.bar);
+    (void)hipFree(c->ps.args);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -632,7 +784,12 @@ int npbcg_conj_grad_device(npbcg_ctx *c, double *rnorm) {
     if (!c) return set_error(NPBCG_ERR_INVALID, "npbcg_conj_grad_device: NULL context");
     NPBCG_HIP(hipSetDevice(c->device));
     NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
-    NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 0, 0, c->stream, nullptr));
+    if (c->engine == NPBCG_ENGINE_PERSISTENT) {
+        int rc = run_persistent(c, 1, 0, 0, 0);
+        if (rc != NPBCG_OK) return rc;
+    } else {
+        NPBCG_HIP(npbcg_dev::enqueue_conj_grad(c->pb, c->cgitmax, 0, 0, c->stream, nullptr));
+    }
     NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
     Scalars s;
     int rc = read_scalars(c, &s);
@@ -642,7 +799,10 @@ int npbcg_conj_grad_device(npbcg_ctx *c, double *rnorm) {
     c->timing.total_ms = ms;
     c->timing.spmv_samples = 0;
     c->timing.spmv_avg_ms = 0.0;
-    c->timing.launches = npbcg_dev::conj_grad_launches(c->pb, c->cgitmax);
+    c->timing.launches = c->engine == NPBCG_ENGINE_PERSISTENT
+                             ? 2
+                             : npbcg_dev::conj_grad_launches(c->pb, c->cgitmax);
+    c->timing.engine = c->engine;
     if (rnorm) *rnorm = s.rnorm;
     clear_error();
     return NPBCG_OK;
@@ -704,6 +864,32 @@ int npbcg_power(npbcg_ctx *c, int niter, double shift, double *zeta_out,
     NPBCG_HIP(hipSetDevice(c->device));
     int rc = ensure_history(c, std::max(niter, 1));
     if (rc != NPBCG_OK) return rc;
+    if (c->engine == NPBCG_ENGINE_PERSISTENT) {
+        // the whole loop in cooperative launches; no SpMV sampling inside one
+        c->device_calls++;
+        NPBCG_HIP(npbcg_dev::launch_reset_scalars(c->pb, shift, c->stream));
+        NPBCG_HIP(hipEventRecord(c->ev_start, c->stream));
+        if (niter > 0) {
+            rc = run_persistent(c, niter, 0, 1, 1);
+            if (rc != NPBCG_OK) return rc;
+        }
+        NPBCG_HIP(hipEventRecord(c->ev_stop, c->stream));
+        NPBCG_HIP(hipStreamSynchronize(c->stream));
+        if (niter > 0 && zeta_out)
+            NPBCG_HIP(hipMemcpy(zeta_out, c->d_hist, sizeof(double) * niter, hipMemcpyDeviceToHost));
+        if (niter > 0 && rnorm_out)
+            NPBCG_HIP(hipMemcpy(rnorm_out, c->d_hist + c->hist_cap, sizeof(double) * niter,
+                                hipMemcpyDeviceToHost));
+        float ms = 0.f;
+        NPBCG_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+        c->timing.total_ms = ms;
+        c->timing.launches = 2 * std::max(niter, 1) + 1;
+        c->timing.spmv_samples = 0;
+        c->timing.spmv_avg_ms = 0.0;
+        c->timing.engine = c->engine;
+        clear_error();
+        return NPBCG_OK;
+    }
     const bool use_graph = c->opts.use_graph != 0;
     if (use_graph) {
         rc = ensure_graph(c);
@@ -749,6 +935,7 @@ int npbcg_power(npbcg_ctx *c, int niter, double shift, double *zeta_out,
     NPBCG_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
     c->timing.total_ms = ms;
     c->timing.launches = launches + 2;
+    c->timing.engine = c->engine;
     c->timing.spmv_samples = 0;
     c->timing.spmv_avg_ms = 0.0;
     if (sample && niter > 0) {
@@ -886,11 +1073,11 @@ int npbcg_describe(const npbcg_ctx *c, char *buf, int len) {
         "\"csr_blocks\": %d, \"block_width\": %d, \"col_blocks\": %d, "
         "\"range_rows\": %d, \"row_ranges\": %d, \"sell_workgroups\": %d, "
         "\"sell_threads\": %d, \"sell_stored\": %lld, \"cgitmax\": %d, "
-        "\"use_graph\": %d, \"fuse_p_update\": %d, \"cg_form\": %d, "
+        "\"use_graph\": %d, \"fuse_p_update\": %d, \"cg_form\": %d, \"engine\": %d, "
         "\"device_name\": \"%s\", \"compute_units\": %d}",
         c->n, c->nnz, c->pb.spmv_variant, c->pb.vec_blocks, c->pb.spmv_blocks, sd.W,
         sd.nblk, sd.R, sd.nrange, sd.grid, sd.block, c->sell_stored, c->cgitmax,
-        c->opts.use_graph, c->pb.fuse_p_update, c->pb.cg_form ? 2 : 1, prop.name,
+        c->opts.use_graph, c->pb.fuse_p_update, c->pb.cg_form ? 2 : 1, c->engine, prop.name,
         prop.multiProcessorCount);
     if (len > 0) {
         std::strncpy(buf, tmp, (size_t)len - 1);
@@ -1285,12 +1472,14 @@ int select_precond(npbcg_ctx *c, int kind) {
     c->pbs.pz = nullptr;
     c->mp.pop = nullptr;
     c->mp.pz = nullptr;
+    c->active_precond = NPBCG_PRECOND_NONE;
     if (kind == NPBCG_PRECOND_NONE) return NPBCG_OK;
     if (kind == NPBCG_PRECOND_JACOBI) {
         int rc = ensure_precond_buffers(c);
         if (rc != NPBCG_OK) return rc;
         c->pbs.minv = c->d_minv;
         c->mp.minv = c->d_minv32;
+        c->active_precond = kind;
         return NPBCG_OK;
     }
     int rc = ensure_row_graph(c);
@@ -1305,6 +1494,7 @@ int select_precond(npbcg_ctx *c, int kind) {
     c->pbs.pz = g.pz;
     c->mp.pop = &g.op;
     c->mp.pz = g.pz32;
+    c->active_precond = kind;
     return NPBCG_OK;
 }
 
@@ -1542,15 +1732,18 @@ int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stat
     if (rc != NPBCG_OK) return rc;
     npbcg_solve_stats s{};
     rc = solve_once(c, mode, rtol, maxit, &s);
-    s.precond = c->precond;
+    s.precond = c->active_precond;
     if (c->precond == NPBCG_PRECOND_LEARNED &&
         (rc == NPBCG_ERR_NOCONV || !(s.relres == s.relres))) {
-        // The learned diagonal did not get there: Jacobi-CG redoes the solve.
-        const float first_ms = (float)s.time_ms;
+        // The learned operator did not get there: Jacobi-CG redoes the solve
+        // from x = 0.  The solve views now point at the Jacobi diagonal, and the
+        // captured solve sequences are keyed by the preconditioner in effect,
+        // so nothing of the learned sequence is replayed.
+        const double first_ms = s.time_ms;
         rc = select_precond(c, NPBCG_PRECOND_JACOBI);
         if (rc != NPBCG_OK) return rc;
         rc = solve_once(c, mode, rtol, maxit, &s);
-        s.precond = NPBCG_PRECOND_JACOBI;
+        s.precond = c->active_precond;
         s.fallback = 1;
         s.time_ms += first_ms;
     }
@@ -1561,9 +1754,10 @@ int npbcg_solve(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stat
 namespace {
 
 // Runs the first stretch of a solve: replays its captured graph when the
-// key (mode, preconditioner, rtol, maxit, batch, ||b||) matches the last
-// capture, else captures `enqueue` anew (or, with use_graph = 0, launches it
-// directly).
+// key (mode, preconditioner in effect, rtol, maxit, batch, ||b||) matches the
+// last capture, else captures `enqueue` anew (or, with use_graph = 0,
+// launches it directly).  The preconditioner in effect decides which
+// buffers the captured kernels read (Jacobi diagonal or sparse operator).
 int run_solve_sequence(npbcg_ctx *c, int mode, double rtol, int maxit, int batch,
                        const std::function<hipError_t()> &enqueue) {
     if (!c->opts.use_graph) {
@@ -1571,7 +1765,7 @@ int run_solve_sequence(npbcg_ctx *c, int mode, double rtol, int maxit, int batch
         return NPBCG_OK;
     }
     auto &sg = c->solve_graph[mode];
-    if (!(sg.ex && sg.kind == c->precond && sg.rtol == rtol && sg.maxit == maxit &&
+    if (!(sg.ex && sg.kind == c->active_precond && sg.rtol == rtol && sg.maxit == maxit &&
           sg.batch == batch && sg.rhs == c->rhs_norm)) {
         if (sg.ex) (void)hipGraphExecDestroy(sg.ex);
         if (sg.g) (void)hipGraphDestroy(sg.g);
@@ -1594,7 +1788,7 @@ int run_solve_sequence(npbcg_ctx *c, int mode, double rtol, int maxit, int batch
         }
         sg.g = g;
         sg.ex = ex;
-        sg.kind = c->precond;
+        sg.kind = c->active_precond;
         sg.rtol = rtol;
         sg.maxit = maxit;
         sg.batch = batch;
@@ -1636,7 +1830,8 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
         // a repeated solve on this matrix launches one step more than the last
         // one needed instead of a full batch (steps past the stop return early
         // but still cost a launch each)
-        int batch = std::min(c->budget[0][0] > 0 ? c->budget[0][0] : batch0, maxit);
+        int *bud = c->budget[0][c->active_precond];
+        int batch = std::min(bud[0] > 0 ? bud[0] : batch0, maxit);
         // first batch: reset, init, `batch` steps, residual (captured once per key)
         auto first_batch = [&]() -> hipError_t {
             hipError_t e = npbcg_dev::launch_reset_scalars(pbs, 0.0, c->stream, rtol, 1);
@@ -1668,7 +1863,10 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
             first = false;
         }
         s.iterations = sc.steps;
-        if (relres <= rtol && s.sweeps == 1) c->budget[0][0] = std::min(batch0, sc.steps + 1);
+        // a converged solve sets the first batch of the next one to its own
+        // step count plus one (at most the default): an easier right-hand side
+        // lowers it, a harder one (which needed a restart) raises it again
+        if (relres <= rtol) bud[0] = std::min(batch0, sc.steps + 1);
     } else {
         const npbcg_dev::MixedProblem &mp = c->mp;
         Fp64Problem inner = pbs;  // same launch shapes, inner scalars
@@ -1677,8 +1875,9 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
         // enqueued before the first host synchronisation (FP32 holds about
         // 1e-6 per sweep, so two reach 1e-10 from x = 0 on well-conditioned
         // systems); later ones go one at a time.  maxit is checked per sweep.
-        const int b1 = c->budget[1][0] > 0 ? c->budget[1][0] : batch0 - 2;
-        const int b2 = c->budget[1][1] > 0 ? c->budget[1][1] : batch0 / 2 + 1;
+        int *bud = c->budget[1][c->active_precond];
+        const int b1 = bud[0] > 0 ? bud[0] : batch0 - 2;
+        const int b2 = bud[1] > 0 ? bud[1] : batch0 / 2 + 1;
         auto sweep = [&](int index) -> hipError_t {
             const int steps = index == 0 ? b1 : index == 1 ? b2 : batch0 / 2 + 1;
             hipError_t e = npbcg_dev::launch_sweep_prepare(pbs.sc, mp.sc, c->rhs_norm, rtol,
@@ -1718,8 +1917,12 @@ int solve_once(npbcg_ctx *c, int mode, double rtol, int maxit, npbcg_solve_stats
         s.sweeps = sc.sweeps;
         s.iterations = inner_steps;
         if (relres <= rtol && s.sweeps == 2) {  // the common case: learn both budgets
-            c->budget[1][0] = std::min(batch0 - 2, std::max(2, sc.sweep1_steps + 1));
-            c->budget[1][1] = std::min(batch0 / 2 + 1, std::max(2, si_last_steps + 1));
+            bud[0] = std::min(batch0 - 2, std::max(2, sc.sweep1_steps + 1));
+            bud[1] = std::min(batch0 / 2 + 1, std::max(2, si_last_steps + 1));
+        } else if (relres <= rtol && s.sweeps == 1) {
+            bud[0] = std::min(batch0 - 2, std::max(2, sc.sweep1_steps + 1));
+        } else if (relres <= rtol) {  // needed more sweeps: back to the defaults
+            bud[0] = bud[1] = 0;
         }
         s.fp32_spmvs = inner_steps;
         s.fp64_spmvs = s.sweeps;

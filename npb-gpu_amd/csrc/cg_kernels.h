@@ -187,6 +187,30 @@ hipError_t launch_sweep_prepare(Scalars *outer, Scalars *inner, double rhs_norm,
 
 size_t scalars_bytes();
 
+// Persistent engine (cg_persistent.hip): one cooperative launch runs whole
+// outer iterations on the SELL layout, with grid barriers between phases.
+struct PersistentState {
+    double *part = nullptr;    // persistent_partials(pb) doubles
+    unsigned *bar = nullptr;   // [0] barrier arrivals, [1] abort word (nonzero: a
+                               // barrier waited past spin_limit; results invalid)
+    long long spin_limit = 0;  // wall-clock ticks one barrier may wait
+    int cgitmax = 25;
+    void *args = nullptr;      // device copy of the kernel's argument block
+                               // (persistent_args_bytes())
+};
+size_t persistent_args_bytes();
+// Workgroups of the persistent kernel that fit on one CU (0 off the SELL layout).
+hipError_t persistent_capacity(const Fp64Problem &pb, int *per_cu);
+int persistent_partials(const Fp64Problem &pb);
+// Grid barriers passed by one outer iteration.
+long long persistent_barriers(int cgitmax);
+// niter outer iterations {conj_grad; rnorm, zeta, xscale} from the context's
+// x (scale_first: x = xscale z of the previous call applied first); record
+// appends zeta and rnorm to the history; final_scale applies x = xscale z after
+// the last one.
+hipError_t launch_persistent(const Fp64Problem &pb, const PersistentState &ps, int niter,
+                             int scale_first, int record, int final_scale, hipStream_t st);
+
 }  // namespace npbcg_dev
 
 #endif

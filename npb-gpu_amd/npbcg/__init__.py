@@ -28,6 +28,7 @@ ERR_MISMATCH = -7
 ERR_NOCONV = -8
 
 ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
+CHECK_FULL, CHECK_SAMPLED = 0, 1
 VEC_X, VEC_Z, VEC_P, VEC_Q, VEC_R = 0, 1, 2, 3, 4
 SPMV_AUTO, SPMV_SEG64, SPMV_SEG32, SPMV_SEG16, SPMV_SEG64_U2, SPMV_TILED = 0, 1, 2, 3, 4, 5
 
@@ -49,7 +50,7 @@ class Opts(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("tile_rows", ctypes.c_int),
                 ("tile_width", ctypes.c_int), ("fuse_p_update", ctypes.c_int),
                 ("solve_batch", ctypes.c_int), ("cg_form", ctypes.c_int),
-                ("reserved", ctypes.c_int * 6)]
+                ("matrix_check", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
 
 
 SOLVE_FP64, SOLVE_MIXED = 0, 1

@@ -21,7 +21,7 @@ def test_exports_every_declared_function(npbcg_lib):
 
 
 def test_abi_version_and_status_strings(npbcg_lib):
-    assert npbcg_lib.LIB.npbcg_abi_version() == 2
+    assert npbcg_lib.LIB.npbcg_abi_version() == 3
     for code in range(0, -9, -1):
         s = npbcg_lib.LIB.npbcg_status_string(code).decode()
         assert s and s != "unknown status"
