@@ -212,6 +212,15 @@ int npbcg_describe(const npbcg_ctx *ctx, char *buf, int len);
  * from HIP events.  Used to measure SpMV bandwidth in isolation. */
 int npbcg_spmv_bench(npbcg_ctx *ctx, int reps, double *avg_ms);
 
+/* The CG SpMV kernel alone (no combine of the per-block row sums), reps
+ * launches back to back as successive CG steps issue them (the LDS-staged
+ * walk alternates direction with the step parity), each bracketed by HIP
+ * events recorded by its own dispatch; *avg_ms gets the mean kernel duration.
+ * precision 0: the FP64 kernel of conj_grad and FP64 solves (the reference's
+ * k3, cg.cu:937-961); 1: the FP32 kernel of the mixed solve's inner CG (needs
+ * the LDS-staged layout; builds the FP32 copy on first use). */
+int npbcg_spmv_kernel_bench(npbcg_ctx *ctx, int precision, int reps, double *avg_ms);
+
 /* Single-system autotuner, in place of the reference's compile-time launch
  * configuration (config/gpu.config:32-42, setparams.cpp:256-332): times the
  * q = A p SpMV of this matrix through temporary contexts (created with *opts,
@@ -227,12 +236,17 @@ int npbcg_tune(int n, const int *rowstr, const int *colidx, const double *a, npb
                const char *cache_path, double *ms);
 
 /* A plain device-to-device copy of `bytes` (16-byte vector loads and
- * stores, grid-stride) repeated reps times on the context's device;
+ * stores, grid-stride, four loads in flight per thread) repeated reps times on the context's device;
  * *avg_ms is the mean duration.  It has no counterpart in the reference: it
  * measures the achievable streaming bandwidth the SpMV is compared with, and
  * calibrates memory-traffic counters with a kernel of known traffic.  The
  * two buffers are allocated and freed inside the call. */
 int npbcg_copy_bench(npbcg_ctx *ctx, long long bytes, int reps, double *avg_ms);
+/* npbcg_copy_bench with a choice of stream: mode 0 the copy (bytes read and
+ * bytes written), mode 1 a read of `bytes` only (four 16-byte loads in flight
+ * per thread, no stores).  A size well above the 256 MB memory-side cache
+ * measures HBM; one that fits measures that cache. */
+int npbcg_stream_bench(npbcg_ctx *ctx, int mode, long long bytes, int reps, double *avg_ms);
 
 /* ---- solve to a tolerance ---------------------------------------------------
  * A x = b from x = 0 until the FP64 true relative residual ||b - A x|| / ||b||
