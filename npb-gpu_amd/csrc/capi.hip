@@ -609,7 +609,8 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
             }
         }
         if (W * 8 > lds_budget || !sorted ||
-            !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, &sm)) {
+            !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, block / npbcg_dev::kWave,
+                                             &sm)) {
             variant = pick_variant(npbcg_dev::kSpmvSeg64, n, nnz, false);
         } else {
             sd.W = sm.W;
@@ -1242,8 +1243,12 @@ int npbcg_tune(int n, const int *rowstr, const int *colidx, const double *a, npb
 
 
 int npbcg_copy_bench(npbcg_ctx *c, long long bytes, int reps, double *avg_ms) {
-    if (!c || bytes < 16 || reps < 1 || !avg_ms)
-        return set_error(NPBCG_ERR_INVALID, "npbcg_copy_bench: invalid argument");
+    return npbcg_stream_bench(c, 0, bytes, reps, avg_ms);
+}
+
+int npbcg_stream_bench(npbcg_ctx *c, int mode, long long bytes, int reps, double *avg_ms) {
+    if (!c || bytes < 16 || reps < 1 || !avg_ms || (mode != 0 && mode != 1))
+        return set_error(NPBCG_ERR_INVALID, "npbcg_stream_bench: invalid argument");
     NPBCG_HIP(hipSetDevice(c->device));
     bytes &= ~15LL;
     void *src = nullptr, *dst = nullptr;
@@ -1251,10 +1256,14 @@ int npbcg_copy_bench(npbcg_ctx *c, long long bytes, int reps, double *avg_ms) {
     if (e == hipSuccess) e = hipMalloc(&dst, (size_t)bytes);
     if (e == hipSuccess) e = hipMemsetAsync(src, 0, (size_t)bytes, c->stream);
     const int blocks = 8 * 256;
-    if (e == hipSuccess) e = npbcg_dev::launch_copy(src, dst, bytes, blocks, c->stream);
+    auto run = [&]() {
+        return mode == 0 ? npbcg_dev::launch_copy(src, dst, bytes, blocks, c->stream)
+                         : npbcg_dev::launch_read(src, bytes, static_cast<double *>(dst), blocks,
+                                                  c->stream);
+    };
+    if (e == hipSuccess) e = run();
     if (e == hipSuccess) e = hipEventRecord(c->ev_start, c->stream);
-    for (int i = 0; i < reps && e == hipSuccess; i++)
-        e = npbcg_dev::launch_copy(src, dst, bytes, blocks, c->stream);
+    for (int i = 0; i < reps && e == hipSuccess; i++) e = run();
     if (e == hipSuccess) e = hipEventRecord(c->ev_stop, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     float ms = 0.f;
@@ -1263,7 +1272,7 @@ int npbcg_copy_bench(npbcg_ctx *c, long long bytes, int reps, double *avg_ms) {
     (void)hipFree(dst);
     if (e != hipSuccess)
         return set_error(e == hipErrorOutOfMemory ? NPBCG_ERR_NOMEM : NPBCG_ERR_DEVICE,
-                         std::string("npbcg_copy_bench: ") + hipGetErrorString(e));
+                         std::string("npbcg_stream_bench: ") + hipGetErrorString(e));
     *avg_ms = ms / reps;
     clear_error();
     return NPBCG_OK;
@@ -1988,6 +1997,56 @@ void npbcg_compat_release(void) {
     std::lock_guard<std::mutex> lk(g_compat_mu);
     npbcg_destroy(g_compat_ctx);
     g_compat_ctx = nullptr;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int npbcg_spmv_kernel_bench(npbcg_ctx *c, int precision, int reps, double *avg_ms) {
+    npbcg_detail::TraceRange trace("npbcg_spmv_kernel_bench");
+    if (!c || reps < 1 || !avg_ms || (precision != 0 && precision != 1))
+        return set_error(NPBCG_ERR_INVALID, "npbcg_spmv_kernel_bench: invalid argument");
+    if (precision == 1 && c->pb.spmv_variant != npbcg_dev::kSpmvTiled)
+        return set_error(NPBCG_ERR_INVALID,
+                         "npbcg_spmv_kernel_bench: the FP32 SpMV needs the LDS-staged layout");
+    NPBCG_HIP(hipSetDevice(c->device));
+    if (precision == 1) {
+        int rc = ensure_mixed(c);
+        if (rc != NPBCG_OK) return rc;
+        // the multiplied vector: x in FP32 (finite values); no stop flag set
+        NPBCG_HIP(npbcg_dev::launch_to_fp32(c->pb.x, c->mp.p, c->n, c->stream));
+        Fp64Problem inner = c->pb;
+        inner.sc = c->mp.sc;
+        NPBCG_HIP(npbcg_dev::launch_clear_done(inner, c->stream, 1));
+    }
+    NPBCG_HIP(npbcg_dev::launch_clear_done(c->pb, c->stream, 1));
+    std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < ev.size() && e == hipSuccess; i++)
+        e = hipEventCreateWithFlags(&ev[i], hipEventDisableSystemFence);
+    auto launch = [&](int step, hipEvent_t e0, hipEvent_t e1) {
+        return precision == 1
+                   ? npbcg_dev::launch_spmv32_kernel(c->pb, c->mp, step, e0, e1, c->stream)
+                   : npbcg_dev::launch_spmv_kernel(c->pb, step, e0, e1, c->stream);
+    };
+    for (int i = 0; i < 2 && e == hipSuccess; i++) e = launch(i, nullptr, nullptr);  // warm
+    for (int i = 0; i < reps && e == hipSuccess; i++) e = launch(i, ev[2 * i], ev[2 * i + 1]);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    double sum = 0.0;
+    for (int i = 0; i < reps && e == hipSuccess; i++) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+        sum += ms;
+    }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess)
+        return set_error(NPBCG_ERR_DEVICE,
+                         std::string("npbcg_spmv_kernel_bench: ") + hipGetErrorString(e));
+    *avg_ms = sum / reps;
+    clear_error();
+    return NPBCG_OK;
 }
 
 }  // extern "C"

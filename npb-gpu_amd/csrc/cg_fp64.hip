@@ -611,12 +611,38 @@ __global__ __launch_bounds__(kBlock) void scale_x_kernel(
         x[j] = s * z[j];
 }
 
-// dst = src over `count` 16-byte elements (bandwidth reference).
+// dst = src over `count` 16-byte elements (bandwidth reference): each thread
+// issues four loads before its four stores.
 __global__ __launch_bounds__(kBlock) void copy_kernel(const double2 *__restrict__ src,
                                                      double2 *__restrict__ dst, long long count) {
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < count;
-         i += (long long)gridDim.x * kBlock)
-        dst[i] = src[i];
+    const long long stride = (long long)gridDim.x * kBlock;
+    long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+    for (; i + 3 * stride < count; i += 4 * stride) {
+        const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                      d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < count; i += stride) dst[i] = src[i];
+}
+
+// Reads `count` 16-byte elements (read-bandwidth reference): four loads in
+// flight per thread; the sum is stored only on an impossible value, which
+// keeps every load live without write traffic.
+__global__ __launch_bounds__(kBlock) void read_kernel(const double2 *__restrict__ src,
+                                                     long long count, double *__restrict__ sink) {
+    const long long stride = (long long)gridDim.x * kBlock;
+    long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+    double s = 0.0;
+    for (; i + 3 * stride < count; i += 4 * stride) {
+        const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                      d = src[i + 3 * stride];
+        s += (a.x + a.y) + (b.x + b.y) + (c.x + c.y) + (d.x + d.y);
+    }
+    for (; i < count; i += stride) s += src[i].x + src[i].y;
+    if (s == -1.2345e300) sink[0] = s;
 }
 
 // Resets the outer-loop bookkeeping.  A kernel rather than a memset so it may
@@ -802,6 +828,11 @@ hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_spmv_kernel(const Fp64Problem &pb, int step, hipEvent_t e0, hipEvent_t e1,
+                              hipStream_t st) {
+    return launch_spmv<0>(pb, st, e0, e1, step);
+}
+
 hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
     hipError_t e = launch_spmv<1>(pb, st);
     if (e != hipSuccess) return e;
@@ -883,6 +914,12 @@ hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
                        hipStream_t st) {
     copy_kernel<<<blocks, kBlock, 0, st>>>(static_cast<const double2 *>(src),
                                            static_cast<double2 *>(dst), bytes / 16);
+    return hipGetLastError();
+}
+
+hipError_t launch_read(const void *src, long long bytes, double *sink, int blocks,
+                       hipStream_t st) {
+    read_kernel<<<blocks, kBlock, 0, st>>>(static_cast<const double2 *>(src), bytes / 16, sink);
     return hipGetLastError();
 }
 

@@ -107,6 +107,10 @@ hipError_t prepare_sell_kernels(int lds_bytes);
 
 // A complete q = A p for every layout (SELL: SpMV then block combine).
 hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st);
+// The CG SpMV kernel alone as CG step `step` launches it (SELL: row
+// partials, walk direction by step parity), bracketed by e0/e1 when non-null.
+hipError_t launch_spmv_kernel(const Fp64Problem &pb, int step, hipEvent_t e0, hipEvent_t e1,
+                              hipStream_t st);
 // r = A z (partials ||x - r||^2, x.z, z.z), then a one-workgroup finish that
 // forms rnorm, zeta, xscale and, with record != 0, appends to the history.
 hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st);
@@ -130,6 +134,10 @@ hipError_t launch_clear_done(const Fp64Problem &pb, hipStream_t st, int end_of_s
 hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream_t st);
 // Copy of bytes (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
+                       hipStream_t st);
+// Read of bytes (a multiple of 16) from src; sink (one double) is written
+// only on an impossible sum.
+hipError_t launch_read(const void *src, long long bytes, double *sink, int blocks,
                        hipStream_t st);
 
 // One conj_grad: init (optionally applying x = xscale z first), cgitmax x
@@ -163,6 +171,10 @@ struct MixedProblem {
     int lds_bytes = 0;
 };
 hipError_t prepare_mixed_kernels(int lds_bytes);
+// The FP32 SELL SpMV kernel of an inner CG step (q = A p with p.q partials,
+// p in mp.p, walk direction by step parity), bracketed by e0/e1.
+hipError_t launch_spmv32_kernel(const Fp64Problem &pb, const MixedProblem &mp, int step,
+                                hipEvent_t e0, hipEvent_t e1, hipStream_t st);
 hipError_t launch_to_fp32(const double *src, float *dst, long long count, hipStream_t st);
 // Inner FP32 CG on A d = b - A x (or b on the first sweep): init and `steps`
 // steps; pb is the solve view (x holds b, z the FP64 iterate, r = A z).

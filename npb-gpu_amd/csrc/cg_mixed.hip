@@ -10,6 +10,8 @@
 // The FP32 matrix shares the FP64 SELL layout's slices and 16-bit columns;
 // only the values are FP32 (6 bytes per stored entry instead of 10).  Kernel
 // structure and the partial-sum handoff are those of the FP64 path.
+#include <hip/hip_ext.h>
+
 #include "cg_device.h"
 #include "cg_kernels.h"
 #include "learned_kernels.h"
@@ -390,6 +392,28 @@ static hipError_t launch_spmv32(const Fp64Problem &pb, const MixedProblem &mp, c
         spmv_sell32_kernel<1024, kSellUnr32, PUPD><<<sd.grid, 1024, mp.lds_bytes, st>>>(
             sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
             pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev);
+    return hipGetLastError();
+}
+
+hipError_t launch_spmv32_kernel(const Fp64Problem &pb, const MixedProblem &mp, int step,
+                                hipEvent_t e0, hipEvent_t e1, hipStream_t st) {
+    const SellDev &sd = pb.sell;
+    const int rev = step & 1;
+    const float *v = mp.p;
+    const float *r_none = nullptr;
+    float *p_none = nullptr;
+    const double *d_none = nullptr;
+    if (sd.block == 512)
+        hipExtLaunchKernelGGL(spmv_sell32_kernel<512, kSellUnr32, false>, dim3(sd.grid), dim3(512),
+                              (uint32_t)mp.lds_bytes, st, e0, e1, 0u, sd.wg_slice, sd.slice_off,
+                              sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq, pb.n, sd.W,
+                              sd.nrange, mp.sc, r_none, p_none, d_none, d_none, pb.vec_blocks, rev);
+    else
+        hipExtLaunchKernelGGL(spmv_sell32_kernel<1024, kSellUnr32, false>, dim3(sd.grid),
+                              dim3(1024), (uint32_t)mp.lds_bytes, st, e0, e1, 0u, sd.wg_slice,
+                              sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part,
+                              mp.part_pq, pb.n, sd.W, sd.nrange, mp.sc, r_none, p_none, d_none,
+                              d_none, pb.vec_blocks, rev);
     return hipGetLastError();
 }
 

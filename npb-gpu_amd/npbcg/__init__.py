@@ -157,10 +157,12 @@ def _load() -> ctypes.CDLL:
                                                  ctypes.POINTER(KernelTime), ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_int), _DP]),
         "npbcg_spmv_bench": (ctypes.c_int, [vp, ctypes.c_int, _DP]),
+        "npbcg_spmv_kernel_bench": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _DP]),
         "npbcg_tune": (ctypes.c_int, [ctypes.c_int, _I32P, _I32P, _F64P, ctypes.c_void_p,
                                       ctypes.c_char_p, _DP]),
         "npbcg_describe": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
         "npbcg_copy_bench": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int, _DP]),
+        "npbcg_stream_bench": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _DP]),
         "npbcg_set_rhs": (ctypes.c_int, [vp, _F64P]),
         "npbcg_solve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(SolveStats)]),
@@ -451,9 +453,26 @@ class Context:
                "npbcg_copy_bench")
         return ms.value
 
+    def stream_bench(self, mode: int, nbytes: int, reps: int = 5) -> float:
+        """Mean ms of a device copy (mode 0) or read-only stream (mode 1) of
+        nbytes (npbcg_stream_bench)."""
+        ms = ctypes.c_double(0.0)
+        _check(LIB.npbcg_stream_bench(self._h, mode, nbytes, reps, ctypes.byref(ms)),
+               "npbcg_stream_bench")
+        return ms.value
+
     def spmv_bench(self, reps: int = 20) -> float:
         ms = ctypes.c_double(0.0)
         _check(LIB.npbcg_spmv_bench(self._h, reps, ctypes.byref(ms)), "npbcg_spmv_bench")
+        return ms.value
+
+    def spmv_kernel_bench(self, precision: int = 0, reps: int = 20) -> float:
+        """Mean event-timed duration (ms) of the CG SpMV kernel alone,
+        launched as successive CG steps launch it (npbcg_spmv_kernel_bench);
+        precision 0 = FP64, 1 = the mixed solve's FP32 kernel."""
+        ms = ctypes.c_double(0.0)
+        _check(LIB.npbcg_spmv_kernel_bench(self._h, precision, reps, ctypes.byref(ms)),
+               "npbcg_spmv_kernel_bench")
         return ms.value
 
 
