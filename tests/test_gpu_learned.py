@@ -158,9 +158,9 @@ def test_train_pipeline_small(gpu):
     o.systems, o.n_min, o.n_max = 12, 2000, 5000
     o.nonzer_min, o.nonzer_max = 7, 11
     o.gen_threads = 4
-    with gpu.Net(seed=8) as net:
+    with gpu.Net(seed=8) as net, gpu.Net(seed=8) as net2:
         st, lh, bh = net.train(o)
-        st2, lh2, bh2 = gpu.Net(seed=8).train(o)    # same seed: same run
+        st2, lh2, bh2 = net2.train(o)    # same seed: same run
     print(st.as_dict())
     assert st.systems == 12 and st.steps == 12 * o.steps_per_system
     assert np.all(np.isfinite(lh)) and np.all(bh > 0)
