@@ -472,6 +472,31 @@ void npbcg_compat_conj_grad(int colidx[], int rowstr[], double x[], double z[],
                             double *rnorm);
 void npbcg_compat_release(void);
 
+/* ---- allocation-free conj_grad -------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the computation of npbcg_conj_grad_device (conj_grad_gpu,
+ * cg.cu:803-846) with no context, on device arrays the caller owns:
+ *   rowstr (n+1), colidx and a (rowstr[n] entries): the CSR, columns ascending;
+ *   x (n+2) read for n+1 entries; z, p, q, r (n+2) written for n+1 entries,
+ *   index n as npbcg_conj_grad leaves it (q = z = 0, r = p = x[n]);
+ *   rnorm: one double of device memory receiving ||x - A z||.
+ * cgitmax CG steps (25 in the reference) run with the 64-lane CSR SpMV in the
+ * reference's step order, so results are bit-identical to npbcg_conj_grad_device
+ * on a context created with spmv_variant = 1.  Work is enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream) and the call returns without
+ * synchronising; it calls no hipMalloc, hipFree or memset, so it can be
+ * captured into a graph.  The partial sums and CG scalars live in a
+ * file-scope device array of NPBCG_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap
+ * on one GPU race.  A launch shape needing more scratch than that (more than
+ * 256 CUs) returns NPBCG_ERR_CAPACITY. */
+#define NPBCG_NOALLOC_SCRATCH_BYTES (160 * 1024)
+int npbcg_conj_grad_device_noalloc(int n, const int *rowstr, const int *colidx,
+                                   const double *a, double *x, double *z, double *p,
+                                   double *q, double *r, int cgitmax, double *rnorm,
+                                   void *stream);
+
 #ifdef __cplusplus
 }
 #endif

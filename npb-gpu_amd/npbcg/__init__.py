@@ -201,6 +201,8 @@ def _load() -> ctypes.CDLL:
         "npbcg_compat_conj_grad": (None, [_I32P, _I32P, _F64P, _F64P, _F64P, _F64P,
                                           _F64P, _F64P, _DP]),
         "npbcg_compat_release": (None, []),
+        "npbcg_conj_grad_device_noalloc": (ctypes.c_int, [ctypes.c_int, vp, vp, vp, vp, vp, vp,
+                                                          vp, vp, ctypes.c_int, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
