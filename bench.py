@@ -165,6 +165,22 @@ def profile_pass():
                           "kernel_config": ctx.describe()}))
 
 
+def profile_pass_solve():
+    """Short run for counter passes on the n=150000 solve matrix: a copy of
+    known size (the calibration kernel), then 10 launches each of the FP64 and
+    the FP32 CG SpMV kernels (npbcg_spmv_kernel_bench)."""
+    import npbcg
+
+    sw = SOLVE_WORKLOAD
+    m = npbcg.makea(sw["n"], sw["nonzer"], sw["shift"], sw["seed"], sw["rcond"])
+    with npbcg.Context(m, device=0, use_graph=0) as ctx:
+        ctx.copy_bench(PROFILE_COPY_BYTES, 3)
+        fp64 = ctx.spmv_kernel_bench(0, 10)
+        fp32 = ctx.spmv_kernel_bench(1, 10)
+        print(json.dumps({"profile_pass_solve": True, "copy_bytes": PROFILE_COPY_BYTES,
+                          "fp64_ms": fp64, "fp32_ms": fp32, "kernel_config": ctx.describe()}))
+
+
 def profile_pass_batch():
     """Short run for counter passes on the batched SpMM: the 64-system batch
     (MIXED, layout 3, 2048-row items) and 20 SpMM launches, every system
@@ -236,6 +252,50 @@ def pmc_summary(fetch_csv, write_csv, out_path):
                    "there, not only HBM reads."}
     with open(out_path, "w") as f:
         json.dump(doc, f, indent=1)
+    print(json.dumps(doc))
+
+
+def pmc_summary_solve(fetch_csv, write_csv, out_path):
+    """FETCH_SIZE / WRITE_SIZE per dispatch of the FP64 and FP32 CG SpMV
+    kernels of the n=150000 counter pass (--profile-pass-solve), scaled as in
+    pmc_summary, beside the bytes the layout streams."""
+    fetch = _pmc_rows(fetch_csv, "FETCH_SIZE")
+    write = _pmc_rows(write_csv, "WRITE_SIZE")
+
+    def rows(d, key, exclude=None):
+        vals = []
+        for k, v in d.items():
+            if key in k and (exclude is None or exclude not in k):
+                vals += v
+        return vals
+
+    expect_kib = PROFILE_COPY_BYTES / 1024.0
+    f_scale = float(np.median(rows(fetch, "copy_kernel"))) / expect_kib
+    w_scale = float(np.median(rows(write, "copy_kernel"))) / expect_kib
+    doc = {"workload": "n=150000 nonzer=15 shift=110 seed=314, SELL column-blocked CG SpMV "
+                       "kernels (FP64 conj_grad/solve kernel, FP32 mixed inner kernel)",
+           "fetch_scale": f_scale, "write_scale": w_scale,
+           "copy_bytes_each_way": PROFILE_COPY_BYTES}
+    for name, key, excl in (("fp64", "spmv_sell_kernel", None), ("fp32", "spmv_sell32_kernel", None)):
+        f = rows(fetch, key, excl)
+        w = rows(write, key, excl)
+        if not f or not w:
+            continue
+        fb = float(np.median(f)) * 1024.0 / f_scale
+        wb = float(np.median(w)) * 1024.0 / w_scale
+        doc[name] = {"dispatches": len(f), "fetch_bytes": fb, "write_bytes": wb,
+                     "bytes_per_launch": fb + wb}
+    doc["commands"] = [
+        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d prof_fetch -o fetch "
+        "-- python3 bench.py --profile-pass-solve",
+        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d prof_write -o write "
+        "-- python3 bench.py --profile-pass-solve",
+        "python3 bench.py --pmc-summary-solve <fetch csv> <write csv> " + out_path]
+    doc["note"] = ("FETCH_SIZE/WRITE_SIZE are L2-to-fabric KiB per dispatch, divided by what they "
+                   "read for a copy of known traffic. Fetched bytes include hits in the 256 MB "
+                   "memory-side cache.")
+    with open(out_path, "w") as fo:
+        json.dump(doc, fo, indent=1)
     print(json.dumps(doc))
 
 
@@ -483,10 +543,14 @@ def main():
                     help="solves per mode in the n=150000 time-to-1e-10 comparison (0 = skip)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short run for rocprofv3 counter passes")
+    ap.add_argument("--profile-pass-solve", action="store_true",
+                    help="short n=150000 SpMV run (FP64 and FP32 kernels) for counter passes")
     ap.add_argument("--profile-pass-batch", action="store_true",
                     help="short batched-SpMM run for rocprofv3 counter passes")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="summarise two counter passes into the traffic file")
+    ap.add_argument("--pmc-summary-solve", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="summarise two n=150000 counter passes (--profile-pass-solve)")
     ap.add_argument("--solve-only", action="store_true",
                     help="only the n=150000 time-to-1e-10 comparison (JSON on stdout)")
     ap.add_argument("--batch-reps", type=int, default=3,
@@ -513,8 +577,14 @@ def main():
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
         return
+    if args.pmc_summary_solve:
+        pmc_summary_solve(*args.pmc_summary_solve)
+        return
     if args.profile_pass:
         profile_pass()
+        return
+    if args.profile_pass_solve:
+        profile_pass_solve()
         return
     if args.profile_pass_batch:
         profile_pass_batch()

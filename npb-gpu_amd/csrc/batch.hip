@@ -322,8 +322,7 @@ bool build_batch_sell(const npbcg_batch *b, int W, const std::vector<int> &rp,
         for (int j = 0; j <= ns; j++) lrs[j] = rp[off + j] - e0;
         for (int k = 0; k < nnz_s; k++) lci[k] = col[(size_t)e0 + k] - off;
         SellBlocks<V> sb;
-        if (!build_sell_blocks(ns, lrs.data(), lci.data(), val.data() + e0, W, kSellWindow,
-                               kSellBlock / 64 /* wavefronts */, &sb) ||
+        if (!build_sell_blocks(ns, lrs.data(), lci.data(), val.data() + e0, W, kSellWindow, &sb) ||
             sb.nblk != h->nb[s] || entries + sb.stored > (long long)INT32_MAX)
             return false;
         h->nrange[s] = sb.nrange;

@@ -610,8 +610,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
             }
         }
         if (W * 8 > lds_budget || !sorted ||
-            !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, block / npbcg_dev::kWave,
-                                             &sm)) {
+            !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, &sm)) {
             variant = pick_variant(npbcg_dev::kSpmvSeg64, n, nnz, false);
         } else {
             sd.W = sm.W;

@@ -59,8 +59,8 @@ void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, i
 
 template <class V>
 bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, int W, int R,
-                int waves, SellBlocks<V> *out) {
-    if (n < 1 || W < 1 || W > 65536 || R < 1 || waves < 1) return false;
+                SellBlocks<V> *out) {
+    if (n < 1 || W < 1 || W > 65536 || R < 1) return false;
     SellBlocks<V> &m = *out;
     m.n = n;
     m.W = W;
@@ -103,15 +103,10 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
         std::vector<RowSeg> segs;
         const int b = wg / m.nrange, rr = wg % m.nrange;
         block_segments(n, rowstr, colidx, W, R, b, rr, &segs);
-        std::vector<int> widths;
-        for (size_t i0 = 0; i0 < segs.size(); i0 += kSliceRows)
-            widths.push_back(segs[i0].hi - segs[i0].lo);
-        const std::vector<int> order = wave_balanced_order(widths, waves);
         long long off = wg_base[wg];
         int s = m.wg_slice[wg];
-        for (size_t k = 0; k < order.size(); k++, s++) {
-            const size_t i0 = (size_t)order[k] * kSliceRows;
-            const int width = widths[order[k]];
+        for (size_t i0 = 0; i0 < segs.size(); i0 += kSliceRows, s++) {
+            const int width = segs[i0].hi - segs[i0].lo;
             m.slice_off[s] = (int)off;
             for (int l = 0; l < kSliceRows && i0 + l < segs.size(); l++) {
                 const RowSeg &rs = segs[i0 + l];
@@ -132,39 +127,14 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
 
 }  // namespace
 
-std::vector<int> wave_balanced_order(const std::vector<int> &width, int waves) {
-    const int nsl = (int)width.size();
-    // the walk's positions of each wavefront, in walk order
-    std::vector<std::vector<int>> pos(waves);
-    for (int k = 0; k < nsl; k++) {
-        const int rd = k / waves, j = k % waves;
-        pos[(rd & 1) ? waves - 1 - j : j].push_back(k);
-    }
-    std::vector<int> idx(nsl);
-    for (int i = 0; i < nsl; i++) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int x, int y) { return width[x] > width[y]; });
-    std::vector<long long> load(waves, 0);
-    std::vector<size_t> used(waves, 0);
-    std::vector<int> order(nsl, -1);
-    for (int i : idx) {
-        int best = -1;
-        for (int w = 0; w < waves; w++)
-            if (used[w] < pos[w].size() && (best < 0 || load[w] < load[best])) best = w;
-        order[pos[best][used[best]++]] = i;
-        load[best] += width[i];
-    }
-    return order;
+bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
+                       int R, SellBlocks<double> *out) {
+    return build_impl(n, rowstr, colidx, a, W, R, out);
 }
 
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
-                       int R, int waves, SellBlocks<double> *out) {
-    return build_impl(n, rowstr, colidx, a, W, R, waves, out);
-}
-
-bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
-                       int R, int waves, SellBlocks<float> *out) {
-    return build_impl(n, rowstr, colidx, a, W, R, waves, out);
+                       int R, SellBlocks<float> *out) {
+    return build_impl(n, rowstr, colidx, a, W, R, out);
 }
 
 }  // namespace npbcg_detail

@@ -10,15 +10,6 @@
 // width are padded with (value 0, column 0).  So one wavefront streams a slice
 // with fully coalesced loads, one row per lane, and no cross-lane reduction.
 // Column indices are 16-bit offsets inside the block (W <= 65536).
-//
-// The kernels hand the slices of a workgroup to its `waves` wavefronts round
-// by round: slice k goes to wavefront j = k % waves in even rounds
-// (rd = k / waves) and to waves - 1 - j in odd ones.  The slices are stored in
-// an order that balances the entries each wavefront streams under that
-// mapping: widest first, each to the wavefront with the fewest entries so far
-// among those with a slot left (the mapping fixes how many slices each one
-// takes).  Rows sorted only within windows leave wide slices at the start of
-// every window, which the plain order would stack on the same wavefronts.
 //   wg_slice[wg] .. wg_slice[wg+1]   slices of workgroup wg
 //   slice_off[s] .. slice_off[s+1]   entries of slice s (multiple of 64)
 //   slice_row[64*s + l]              row (global) of lane l, -1 for padding
@@ -45,17 +36,12 @@ struct SellBlocks {
     std::vector<uint16_t> col;   // stored
 };
 
-// Builds the copy for workgroups of `waves` wavefronts; the CSR must have
-// ascending columns within each row.  Returns false if W > 65536, waves < 1
-// or the sizes overflow 32-bit offsets.
+// Builds the copy; the CSR must have ascending columns within each row.
+// Returns false if W > 65536 or the sizes overflow 32-bit offsets.
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
-                       int W, int R, int waves, SellBlocks<double> *out);
+                       int W, int R, SellBlocks<double> *out);
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
-                       int W, int R, int waves, SellBlocks<float> *out);
-
-// Storage order of a workgroup's slices (position k of the wavefront walk
-// holds slice order[k] of the window order) given their widths.
-std::vector<int> wave_balanced_order(const std::vector<int> &width, int waves);
+                       int W, int R, SellBlocks<float> *out);
 
 }  // namespace npbcg_detail
 
