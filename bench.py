@@ -183,16 +183,20 @@ def profile_pass_solve():
 
 def profile_pass_batch():
     """Short run for counter passes on the batched SpMM: the 64-system batch
-    (MIXED, layout 3, 2048-row items) and 20 SpMM launches, every system
-    running."""
+    (layout 3, 2048-row items), 20 SpMM launches in MIXED mode (the FP32
+    kernel) and 20 in FP64 mode, every system running, after a copy of known
+    size (the calibration kernel)."""
     import npbcg
 
     bw = BATCH_WORKLOAD
     mats = npbcg.make_many(batch_systems(bw))
-    with npbcg.Batch(mats, nrhs=bw["nrhs"], mode=npbcg.SOLVE_MIXED, rows_per_tile=2048,
-                     layout=3) as bt:
-        ms = bt.spmm_bench(20)
-    print(json.dumps({"profile_pass_batch": True, "spmm_ms": round(ms, 4)}))
+    out = {"profile_pass_batch": True, "copy_bytes": PROFILE_COPY_BYTES}
+    with npbcg.Context(mats[0], device=0, use_graph=0) as ctx:
+        ctx.copy_bench(PROFILE_COPY_BYTES, 3)
+    for name, mode in (("fp32", npbcg.SOLVE_MIXED), ("fp64", npbcg.SOLVE_FP64)):
+        with npbcg.Batch(mats, nrhs=bw["nrhs"], mode=mode, rows_per_tile=2048, layout=3) as bt:
+            out[name + "_spmm_ms"] = round(bt.spmm_bench(20), 4)
+    print(json.dumps(out))
 
 
 def _pmc_rows(path, counter):

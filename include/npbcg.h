@@ -244,8 +244,8 @@ int npbcg_tune(int n, const int *rowstr, const int *colidx, const double *a, npb
  * two buffers are allocated and freed inside the call. */
 int npbcg_copy_bench(npbcg_ctx *ctx, long long bytes, int reps, double *avg_ms);
 /* npbcg_copy_bench with a choice of stream: mode 0 the copy (bytes read and
- * bytes written), mode 1 a read of `bytes` only (four 16-byte loads in flight
- * per thread, no stores).  A size well above the 256 MB memory-side cache
+ * bytes written), mode 1 a read of `bytes` only (one 1024-thread workgroup per
+ * CU, eight 16-byte loads in flight per thread, no stores).  A size well above the 256 MB memory-side cache
  * measures HBM; one that fits measures that cache. */
 int npbcg_stream_bench(npbcg_ctx *ctx, int mode, long long bytes, int reps, double *avg_ms);
 

@@ -552,8 +552,7 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
 
     int variant = pick_variant(c->opts.spmv_variant, n, nnz, sorted);
     const int rpb = npbcg_dev::spmv_rows_per_block(variant);
-    const int vec_blocks = std::max(1, std::min((n + 1 + npbcg_dev::kBlock - 1) /
-                                                    npbcg_dev::kBlock, 8 * cus));
+    const int vec_blocks = npbcg_dev::vector_blocks(n, cus);
     const int spmv_blocks = std::max(1, std::min((n + rpb - 1) / rpb, 4 * cus));
 
     // Column-blocked SELL copy for the LDS-staged SpMV.  One column block of
@@ -640,18 +639,22 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     if (variant == npbcg_dev::kSpmvTiled) {
         CREATE_HIP(npbcg_dev::prepare_sell_kernels(sd.lds_bytes));
         const size_t nwg1 = sm.wg_slice.size(), nsl1 = sm.slice_off.size();
-        const size_t idx = nwg1 + nsl1 + sm.slice_row.size();
+        const size_t nrow = sm.slice_row.size();
+        const size_t idx = nwg1 + nsl1 + nrow + sm.row_slot.size();
         CREATE_HIP(hipMalloc(&c->d_sell_idx, sizeof(int) * idx));
         CREATE_HIP(hipMalloc(&c->d_sell_val, sizeof(double) * std::max<size_t>(1, sm.val.size())));
         CREATE_HIP(hipMalloc(&c->d_sell_col,
                              sizeof(uint16_t) * std::max<size_t>(1, sm.col.size())));
-        CREATE_HIP(hipMalloc(&c->d_rows_part, sizeof(double) * (size_t)sm.nblk * n));
+        // one block sum per slot (64 per slice), stored by slot
+        CREATE_HIP(hipMalloc(&c->d_rows_part, sizeof(double) * std::max<size_t>(1, nrow)));
         CREATE_HIP(hipMemcpy(c->d_sell_idx, sm.wg_slice.data(), sizeof(int) * nwg1,
                              hipMemcpyHostToDevice));
         CREATE_HIP(hipMemcpy(c->d_sell_idx + nwg1, sm.slice_off.data(), sizeof(int) * nsl1,
                              hipMemcpyHostToDevice));
         CREATE_HIP(hipMemcpy(c->d_sell_idx + nwg1 + nsl1, sm.slice_row.data(),
-                             sizeof(int) * sm.slice_row.size(), hipMemcpyHostToDevice));
+                             sizeof(int) * nrow, hipMemcpyHostToDevice));
+        CREATE_HIP(hipMemcpy(c->d_sell_idx + nwg1 + nsl1 + nrow, sm.row_slot.data(),
+                             sizeof(int) * sm.row_slot.size(), hipMemcpyHostToDevice));
         if (!sm.val.empty()) {
             CREATE_HIP(hipMemcpy(c->d_sell_val, sm.val.data(), sizeof(double) * sm.val.size(),
                                  hipMemcpyHostToDevice));
@@ -661,6 +664,8 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
         sd.wg_slice = c->d_sell_idx;
         sd.slice_off = c->d_sell_idx + nwg1;
         sd.slice_row = c->d_sell_idx + nwg1 + nsl1;
+        sd.row_slot = c->d_sell_idx + nwg1 + nsl1 + nrow;
+        sd.slots = (long long)nrow;
         sd.val = c->d_sell_val;
         sd.col = c->d_sell_col;
         sd.rows_part = c->d_rows_part;
@@ -1255,10 +1260,13 @@ int npbcg_stream_bench(npbcg_ctx *c, int mode, long long bytes, int reps, double
     hipError_t e = hipMalloc(&src, (size_t)bytes);
     if (e == hipSuccess) e = hipMalloc(&dst, (size_t)bytes);
     if (e == hipSuccess) e = hipMemsetAsync(src, 0, (size_t)bytes, c->stream);
-    const int blocks = 8 * 256;
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, c->device);
+    const int cus = (e == hipSuccess && prop.multiProcessorCount > 0) ? prop.multiProcessorCount
+                                                                       : 256;
     auto run = [&]() {
-        return mode == 0 ? npbcg_dev::launch_copy(src, dst, bytes, blocks, c->stream)
-                         : npbcg_dev::launch_read(src, bytes, static_cast<double *>(dst), blocks,
+        return mode == 0 ? npbcg_dev::launch_copy(src, dst, bytes, 8 * cus, c->stream)
+                         : npbcg_dev::launch_read(src, bytes, static_cast<double *>(dst), cus,
                                                   c->stream);
     };
     if (e == hipSuccess) e = run();

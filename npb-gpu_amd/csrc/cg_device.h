@@ -44,6 +44,15 @@ __device__ __forceinline__ double segment_sum(double v) {
     return v;
 }
 
+// The chunk of kBlock rows (per grid sweep) that workgroup bid of nb takes
+// in the kernels that read block row sums by slot.  With nb a multiple of 8,
+// the workgroups dispatched to one XCD (every 8th) take consecutive chunks,
+// so the slots of their rows, which lie in runs of slices per sort window,
+// are fetched into that XCD's L2 once rather than by every XCD.
+__device__ __forceinline__ int xcd_chunk(int bid, int nb) {
+    return (nb & 7) ? bid : (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
 // Copies count elements src[0..count) to LDS dst[0..count) with BLOCK
 // threads.  Every thread first issues its loads for the first FILL * BLOCK
 // elements (FILL in flight at once, held in registers), then writes them: the

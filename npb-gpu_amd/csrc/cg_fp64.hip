@@ -25,6 +25,8 @@
 // workgroup computes the same bits).  Kernel boundaries give the ordering.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "cg_device.h"
 #include "cg_kernels.h"
 #include "learned_kernels.h"
@@ -119,11 +121,14 @@ __global__ __launch_bounds__(kBlock) void spmv_csr_kernel(
 // accumulates its own row down the slice (value and 16-bit column loads are
 // contiguous across the wavefront; UNR entries are in flight per lane), so no
 // cross-lane reduction is needed.  The row's sum over this block goes to
-// part[b * n + row]; update_zr (or resid_combine) adds the blocks in order
+// the lane's slot, part[64 * slice + lane], so each slice stores its 64 sums
+// as one coalesced run (scattered 8-byte stores by row were written back as
+// partial lines, several times their size); update_zr (or resid_combine)
+// finds row j's sums through row_slot and adds the blocks in order
 // b = 0, 1, ... .  The random 8-byte gathers that bound the CSR kernel (one
 // 128-byte L2 line per gather) become LDS reads, and the vector crosses from
 // L2 into LDS once per workgroup.  For q = A p the workgroup also forms its
-// partial of p.q = sum_b sum_rows p[row] * part[b][row].
+// partial of p.q = sum_b sum_rows p[row] * (block b's sum of the row).
 // ---------------------------------------------------------------------------
 
 // PUPD: the input vector is p_new = r + beta p_old, built here instead of by
@@ -219,7 +224,6 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
         if ((wb & 1) && threadIdx.x == 0) vblk[wb - 1] = v[c0 + wb - 1];
     }
     __syncthreads();
-    double *part_b = rows_part + (size_t)b * n;
     for (int i = 0; cur >= 0; i++) {
         const int k = cur, width = cwidth, base = cbase, row = nrow;
         const int nxt = slice_at(i + 1);
@@ -256,7 +260,8 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
             nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
             if (xwidth > 0) sell_chunk<double, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
         }
-        if (row >= 0) part_b[row] = acc;
+        // the slice's 64 block sums, one coalesced run in slot order
+        rows_part[(size_t)(s0 + k) * kWave + lane] = acc;
         if constexpr (MODE == 0) {
             const double d = segment_sum<64>(pr * acc);
             if (lane == 0) slice_dot[k] = d;
@@ -273,13 +278,16 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     }
 }
 
-// y = sum over blocks of the SELL SpMV's row partials (blocks in order).
+// y = sum over blocks of the SELL SpMV's row partials (blocks in order),
+// each found at its slot.
 __global__ __launch_bounds__(kBlock) void combine_rows_kernel(
-    const double *__restrict__ rows_part, int nblk, double *__restrict__ y, int n) {
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        double s = rows_part[j];
+    const double *__restrict__ rows_part, const int *__restrict__ row_slot, int nblk,
+    double *__restrict__ y, int n) {
+    for (int j = xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < n;
+         j += gridDim.x * kBlock) {
+        double s = rows_part[row_slot[j]];
 #pragma unroll 8
-        for (int b = 1; b < nblk; b++) s += rows_part[(size_t)b * n + j];
+        for (int b = 1; b < nblk; b++) s += rows_part[row_slot[(size_t)b * n + j]];
         y[j] = s;
     }
 }
@@ -287,15 +295,16 @@ __global__ __launch_bounds__(kBlock) void combine_rows_kernel(
 // r = sum over blocks of the residual SpMV's row partials, with partials of
 // ||x - r||^2, x.z and z.z (stride gridDim.x).
 __global__ __launch_bounds__(kBlock) void resid_combine_kernel(
-    const double *__restrict__ rows_part, int nblk, const double *__restrict__ x,
-    const double *__restrict__ z, double *__restrict__ r, int n, double *__restrict__ res,
-    const Scalars *__restrict__ sc) {
+    const double *__restrict__ rows_part, const int *__restrict__ row_slot, int nblk,
+    const double *__restrict__ x, const double *__restrict__ z, double *__restrict__ r, int n,
+    double *__restrict__ res, const Scalars *__restrict__ sc) {
     if (sc->skip_resid) return;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        double rj = rows_part[j];
+    for (int j = xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < n;
+         j += gridDim.x * kBlock) {
+        double rj = rows_part[row_slot[j]];
 #pragma unroll 8
-        for (int b = 1; b < nblk; b++) rj += rows_part[(size_t)b * n + j];
+        for (int b = 1; b < nblk; b++) rj += rows_part[row_slot[(size_t)b * n + j]];
         r[j] = rj;
         const double xv = x[j], zv = z[j];
         const double dlt = xv - rj;
@@ -318,25 +327,30 @@ __global__ __launch_bounds__(kBlock) void resid_combine_kernel(
 // rho comes from the partials of the previous r.r (or of x.x for the first
 // step), p.q from the SpMV's partials.
 // With nblk > 0, q is first assembled from the SELL SpMV's per-block row
-// partials (blocks added in order) and stored.
+// partials (blocks added in order, each found at its slot) and stored.
 __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     const double *__restrict__ p, double *__restrict__ q, double *__restrict__ z,
     double *__restrict__ r, int n, const double *__restrict__ rho_part, int n_rho,
     const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
-    const double *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
-    const double *__restrict__ minv, const double *__restrict__ tol_part) {
+    const double *__restrict__ rows_part, const int *__restrict__ row_slot, int nblk,
+    Scalars *sc, int first_step, const double *__restrict__ minv,
+    const double *__restrict__ tol_part) {
     if (sc->done) return;
     // This thread's first row (its block partials of q, p, z, r) is loaded
     // before the scalar reductions, which it does not depend on, so the two
     // memory round trips overlap; the sums keep their order.
-    const int j0 = blockIdx.x * kBlock + threadIdx.x;
+    const int j0 = xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     constexpr int kPre = 8;  // block partials of q prefetched
     double qb[kPre], p0 = 0.0, z0 = 0.0, r0 = 0.0, m0 = 0.0;
     if (j0 < n) {
         if (nblk > 0) {
+            int sl[kPre];
 #pragma unroll
             for (int b = 0; b < kPre; b++)
-                if (b < nblk) qb[b] = rows_part[(size_t)b * n + j0];
+                if (b < nblk) sl[b] = row_slot[(size_t)b * n + j0];
+#pragma unroll
+            for (int b = 0; b < kPre; b++)
+                if (b < nblk) qb[b] = rows_part[sl[b]];
         } else {
             qb[0] = q[j0];
         }
@@ -370,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
             for (int b = 1; b < kPre; b++)
                 if (b < nblk) qj += qb[b];
 #pragma unroll 8
-            for (int b = kPre; b < nblk; b++) qj += rows_part[(size_t)b * n + j0];
+            for (int b = kPre; b < nblk; b++) qj += rows_part[row_slot[(size_t)b * n + j0]];
             q[j0] = qj;
         } else {
             qj = qb[0];
@@ -383,9 +397,9 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     for (int j = j0 + gridDim.x * kBlock; j < n; j += gridDim.x * kBlock) {
         double qj;
         if (nblk > 0) {
-            qj = rows_part[j];
+            qj = rows_part[row_slot[j]];
 #pragma unroll 8
-            for (int b = 1; b < nblk; b++) qj += rows_part[(size_t)b * n + j];
+            for (int b = 1; b < nblk; b++) qj += rows_part[row_slot[(size_t)b * n + j]];
             q[j] = qj;
         } else {
             qj = q[j];
@@ -452,15 +466,20 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
     double *__restrict__ p, double *__restrict__ q, double *__restrict__ z,
     double *__restrict__ r, int n, const double *__restrict__ rho_part, int n_rho,
     const double *__restrict__ rs_part, int n_rs, double *__restrict__ rr_out,
-    const double *__restrict__ rows_part, int nblk, Scalars *sc, int step) {
+    const double *__restrict__ rows_part, const int *__restrict__ row_slot, int nblk,
+    Scalars *sc, int step) {
     // this thread's first row, loaded ahead of the reductions
-    const int j0 = blockIdx.x * kBlock + threadIdx.x;
+    const int j0 = xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     constexpr int kPre = 8;
     double sb[kPre], p0 = 0.0, q0 = 0.0, z0 = 0.0, r0 = 0.0;
     if (j0 < n) {
+        int sl[kPre];
 #pragma unroll
         for (int b = 0; b < kPre; b++)
-            if (b == 0 || b < nblk) sb[b] = rows_part[(size_t)b * n + j0];
+            if (b == 0 || b < nblk) sl[b] = row_slot[(size_t)b * n + j0];
+#pragma unroll
+        for (int b = 0; b < kPre; b++)
+            if (b == 0 || b < nblk) sb[b] = rows_part[sl[b]];
         p0 = p[j0];
         q0 = q[j0];
         z0 = z[j0];
@@ -494,15 +513,15 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
             for (int b = 1; b < kPre; b++)
                 if (b < nblk) sj += sb[b];
 #pragma unroll 8
-            for (int b = kPre; b < nblk; b++) sj += rows_part[(size_t)b * n + j];
+            for (int b = kPre; b < nblk; b++) sj += rows_part[row_slot[(size_t)b * n + j]];
             pj = p0;
             qj = q0;
             zj = z0;
             rj = r0;
         } else {
-            sj = rows_part[j];
+            sj = rows_part[row_slot[j]];
 #pragma unroll 8
-            for (int b = 1; b < nblk; b++) sj += rows_part[(size_t)b * n + j];
+            for (int b = 1; b < nblk; b++) sj += rows_part[row_slot[(size_t)b * n + j]];
             pj = p[j];
             qj = q[j];
             zj = z[j];
@@ -628,18 +647,25 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const double2 *__restrict_
     for (; i < count; i += stride) dst[i] = src[i];
 }
 
-// Reads `count` 16-byte elements (read-bandwidth reference): four loads in
-// flight per thread; the sum is stored only on an impossible value, which
-// keeps every load live without write traffic.
-__global__ __launch_bounds__(kBlock) void read_kernel(const double2 *__restrict__ src,
-                                                     long long count, double *__restrict__ sink) {
-    const long long stride = (long long)gridDim.x * kBlock;
-    long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+// Reads `count` 16-byte elements (read-bandwidth reference): 1024-thread
+// workgroups, eight loads in flight per thread, the grid sweeping the buffer
+// in order (of the shapes measured on MI355X, the fastest read from both HBM
+// and the memory-side cache); the sum is stored only on an impossible value,
+// which keeps every load live without write traffic.
+constexpr int kReadBlock = 1024;
+constexpr int kReadUnroll = 8;
+__global__ __launch_bounds__(kReadBlock) void read_kernel(const double2 *__restrict__ src,
+                                                          long long count,
+                                                          double *__restrict__ sink) {
+    const long long stride = (long long)gridDim.x * kReadBlock;
+    long long i = blockIdx.x * (long long)kReadBlock + threadIdx.x;
     double s = 0.0;
-    for (; i + 3 * stride < count; i += 4 * stride) {
-        const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
-                      d = src[i + 3 * stride];
-        s += (a.x + a.y) + (b.x + b.y) + (c.x + c.y) + (d.x + d.y);
+    for (; i + (kReadUnroll - 1) * stride < count; i += kReadUnroll * stride) {
+        double2 v[kReadUnroll];
+#pragma unroll
+        for (int u = 0; u < kReadUnroll; u++) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < kReadUnroll; u++) s += v[u].x + v[u].y;
     }
     for (; i < count; i += stride) s += src[i].x + src[i].y;
     if (s == -1.2345e300) sink[0] = s;
@@ -789,6 +815,11 @@ hipError_t set_sell_lds_t(int bytes) {
 
 }  // namespace
 
+int vector_blocks(int n, int cus) {
+    const int want = std::max(1, std::min((n + 1 + kBlock - 1) / kBlock, 8 * cus));
+    return want >= 8 ? (want + 7) & ~7 : want;
+}
+
 int spmv_grid(const Fp64Problem &pb) {
     return pb.spmv_variant == kSpmvTiled ? pb.sell.grid : pb.spmv_blocks;
 }
@@ -823,8 +854,8 @@ hipError_t launch_spmv_q(const Fp64Problem &pb, hipStream_t st) {
     pq.cg_form = 0;  // q = A p, whatever form the CG loop uses
     hipError_t e = launch_spmv<0>(pq, st);
     if (e != hipSuccess || pb.spmv_variant != kSpmvTiled) return e;
-    combine_rows_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.sell.rows_part, pb.sell.nblk,
-                                                          pb.q, pb.n);
+    combine_rows_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(pb.sell.rows_part, pb.sell.row_slot,
+                                                          pb.sell.nblk, pb.q, pb.n);
     return hipGetLastError();
 }
 
@@ -838,7 +869,8 @@ hipError_t launch_resid(const Fp64Problem &pb, int record, hipStream_t st) {
     if (e != hipSuccess) return e;
     if (pb.spmv_variant == kSpmvTiled) {
         resid_combine_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
-            pb.sell.rows_part, pb.sell.nblk, pb.x, pb.z, pb.r, pb.n, pb.part.res, pb.sc);
+            pb.sell.rows_part, pb.sell.row_slot, pb.sell.nblk, pb.x, pb.z, pb.r, pb.n,
+            pb.part.res, pb.sc);
         e = noted(pb, hipGetLastError(), "resid_combine_kernel", kBlock, st);
         if (e != hipSuccess) return e;
     }
@@ -865,7 +897,7 @@ hipError_t launch_update_zr(const Fp64Problem &pb, int step, hipStream_t st,
     update_zr_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
         p_of_step(pb, step), pb.q, pb.z, pb.r, pb.n,
         pb.pop ? pb.part.rz[(step + 1) & 1] : rho_source(pb, step), pb.vec_blocks, pb.part.pq,
-        spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part,
+        spmv_grid(pb), pb.part.rr[step & 1], pb.sell.rows_part, pb.sell.row_slot,
         pb.spmv_variant == kSpmvTiled ? pb.sell.nblk : 0, pb.sc,
         (step == 0 && sets_tol) ? 1 : 0, pb.minv, pb.pop ? pb.part.xx : nullptr);
     return noted(pb, hipGetLastError(), "update_zr_kernel", kBlock, st);
@@ -919,7 +951,8 @@ hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
 
 hipError_t launch_read(const void *src, long long bytes, double *sink, int blocks,
                        hipStream_t st) {
-    read_kernel<<<blocks, kBlock, 0, st>>>(static_cast<const double2 *>(src), bytes / 16, sink);
+    read_kernel<<<blocks, kReadBlock, 0, st>>>(static_cast<const double2 *>(src), bytes / 16,
+                                               sink);
     return hipGetLastError();
 }
 
@@ -941,7 +974,8 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
             if (e != hipSuccess) break;
             update_cg_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
                 pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, it), pb.vec_blocks, pb.part.pq,
-                spmv_grid(pb), pb.part.rr[it & 1], pb.sell.rows_part, pb.sell.nblk, pb.sc, it);
+                spmv_grid(pb), pb.part.rr[it & 1], pb.sell.rows_part, pb.sell.row_slot,
+                pb.sell.nblk, pb.sc, it);
             e = noted(pb, hipGetLastError(), "update_cg_kernel", kBlock, st);
         }
         if (e == hipSuccess && cgitmax > 0) e = launch_update_p(pb, cgitmax - 1, st);

@@ -30,7 +30,9 @@ struct SellDev {
     const int *slice_row = nullptr;
     const double *val = nullptr;
     const uint16_t *col = nullptr;
-    double *rows_part = nullptr;  // nblk * n row sums, one set per column block
+    const int *row_slot = nullptr;  // nblk * n: slot of row j in column block b
+    double *rows_part = nullptr;  // one block row sum per slot, stored in slot order
+    long long slots = 0;          // 64 per slice
     int W = 0, nblk = 0, R = 0, nrange = 0;
     int grid = 0;       // nblk * nrange workgroups
     int block = 1024;   // threads per workgroup: 1024 or 512
@@ -92,6 +94,12 @@ struct Fp64Problem {
 // Workgroups launched by the SpMV of this problem (= partials it writes).
 int spmv_grid(const Fp64Problem &pb);
 
+// Workgroups of the element-wise kernels for order n on `cus` CUs: one per
+// 256 entries, at most 8 per CU, rounded up to a multiple of the 8 XCDs so
+// the kernels that combine block row sums can give each XCD a contiguous
+// run of rows (xcd_chunk in cg_device.h).
+int vector_blocks(int n, int cus);
+
 // Optional HIP events bracketing the first `count` SpMV launches of a
 // conj_grad call (used to time the SpMV kernel inside a timed run).
 struct SpmvProbe {
@@ -135,8 +143,8 @@ hipError_t launch_fill(double *v, int count, double value, int blocks, hipStream
 // Copy of bytes (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void *src, void *dst, long long bytes, int blocks,
                        hipStream_t st);
-// Read of bytes (a multiple of 16) from src; sink (one double) is written
-// only on an impossible sum.
+// Read of bytes (a multiple of 16) from src with `blocks` 1024-thread
+// workgroups; sink (one double) is written only on an impossible sum.
 hipError_t launch_read(const void *src, long long bytes, double *sink, int blocks,
                        hipStream_t st);
 
@@ -162,7 +170,7 @@ struct MixedProblem {
     float *pz = nullptr;
     float *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;  // n+2 each
     float *p_alt = nullptr;       // second p buffer of the fused p update
-    float *rows_part = nullptr;   // nblk * n per-block row sums
+    float *rows_part = nullptr;   // nblk * n per-block row sums, by row
     double *part_xx = nullptr;    // r0.r0 partials of an inner start [vec_blocks]
     double *part_rr[2] = {nullptr, nullptr};  // r.r partials [vec_blocks]
     double *part_rz[2] = {nullptr, nullptr};  // r.M^-1 r partials (sparse preconditioner)

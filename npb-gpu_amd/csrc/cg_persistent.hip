@@ -10,8 +10,8 @@
 // and the new r.r partials.  Phases of one outer iteration (G workgroups, one
 // per CU on the column-blocked SELL layout, all resident: cooperative launch):
 //   SpMV(k)    stage the k-th multiplied vector's column block in LDS, stream
-//              the workgroup's slices, row sums to rows_part[b][row], r.s
-//              partial; at k = 0 the vector is x, or xscale * z (the previous
+//              the workgroup's slices, row sums to their slots (64 per slice,
+//              one coalesced run), r.s partial; at k = 0 the vector is x, or xscale * z (the previous
 //              iteration's normalisation, cg.cu:479-481) which the row-range-0
 //              workgroups also store into x with its x.x partial
 //   barrier
@@ -49,7 +49,8 @@ struct PersistArgs {
     const int *wg_slice, *slice_off, *slice_row;
     const double *val;
     const uint16_t *col;
-    double *rows_part;  // nblk * n
+    const int *row_slot;  // nblk * n: slot of row j in column block b
+    double *rows_part;    // block row sums by slot
     int n, W, nrange, nblk;
     double *x, *z, *p, *q, *r;
     double *part_rs;     // [G] r.s of the SpMV phase
@@ -89,6 +90,7 @@ __device__ __forceinline__ PersistArgs phase_args(const PersistArgs *args) {
     a.slice_row = global_ptr(c->slice_row);
     a.val = global_ptr(c->val);
     a.col = global_ptr(c->col);
+    a.row_slot = global_ptr(c->row_slot);
     a.rows_part = global_ptr(c->rows_part);
     a.n = c->n;
     a.W = c->W;
@@ -220,11 +222,11 @@ __device__ __forceinline__ void spmv_phase(const PersistArgs &a, const double *s
     SellWalk<double, BLOCK, kPersistUnr> walk;
     walk.begin(a.wg_slice, a.slice_off, a.slice_row, a.val, a.col, wg, rev);
     __syncthreads();
-    double *part_b = a.rows_part + (size_t)b * a.n;
+    double *part_wg = a.rows_part + (size_t)walk.s0 * kWave;
     walk.run(
         vblk, [&](int row) -> double { return (dot && row >= 0) ? scale * src[row] : 0.0; },
         [&](int k, int row, double acc, double pr) {
-            if (row >= 0) part_b[row] = acc;
+            part_wg[(size_t)k * kWave + (threadIdx.x & (kWave - 1))] = acc;
             if (dot) {
                 const double d = segment_sum<64>(pr * acc);
                 if ((threadIdx.x & (kWave - 1)) == 0) slice_dot[k] = d;
@@ -291,10 +293,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                 }
                 const double alpha = uniform(rho / delta);
                 double acc = 0.0;
-                for (int j = blockIdx.x * BLOCK + phase_tid(); j < n; j += gs) {
-                    double sj = a.rows_part[j];
+                for (int j = xcd_chunk(blockIdx.x, G) * BLOCK + phase_tid(); j < n; j += gs) {
+                    double sj = a.rows_part[a.row_slot[j]];
 #pragma unroll 8
-                    for (int bb = 1; bb < nblk; bb++) sj += a.rows_part[(size_t)bb * n + j];
+                    for (int bb = 1; bb < nblk; bb++)
+                        sj += a.rows_part[a.row_slot[(size_t)bb * n + j]];
                     double pj, qj, zj, rj;
                     if (k == 0) {  // p = r = x, q = s, z = 0 (cg.cu:671-686)
                         rj = a.x[j];
@@ -335,10 +338,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             const PersistArgs a = phase_args(args);
             const int n = a.n, nblk = a.nblk;
             double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-            for (int j = blockIdx.x * BLOCK + phase_tid(); j < n; j += gs) {
-                double rj = a.rows_part[j];
+            for (int j = xcd_chunk(blockIdx.x, G) * BLOCK + phase_tid(); j < n; j += gs) {
+                double rj = a.rows_part[a.row_slot[j]];
 #pragma unroll 8
-                for (int bb = 1; bb < nblk; bb++) rj += a.rows_part[(size_t)bb * n + j];
+                for (int bb = 1; bb < nblk; bb++) rj += a.rows_part[a.row_slot[(size_t)bb * n + j]];
                 a.r[j] = rj;
                 const double xv = a.x[j], zv = a.z[j];
                 const double dlt = xv - rj;
@@ -436,6 +439,7 @@ hipError_t launch_persistent(const Fp64Problem &pb, const PersistentState &ps, i
     a.slice_row = (d.slice_row);
     a.val = (d.val);
     a.col = (d.col);
+    a.row_slot = (d.row_slot);
     a.rows_part = (d.rows_part);
     a.n = pb.n;
     a.W = d.W;

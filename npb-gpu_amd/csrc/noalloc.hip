@@ -86,8 +86,7 @@ extern "C" int npbcg_conj_grad_device_noalloc(int n, const int *rowstr, const in
 
     // launch shape of a context with spmv_variant 1 (npbcg_create)
     const int rpb = npbcg_dev::spmv_rows_per_block(npbcg_dev::kSpmvSeg64);
-    const int vec_blocks =
-        std::max(1, std::min((n + 1 + npbcg_dev::kBlock - 1) / npbcg_dev::kBlock, 8 * cus));
+    const int vec_blocks = npbcg_dev::vector_blocks(n, cus);
     const int spmv_blocks = std::max(1, std::min((n + rpb - 1) / rpb, 4 * cus));
     const size_t part_doubles = 7 * (size_t)vec_blocks + 4 * (size_t)spmv_blocks;
     const size_t sc_off = (part_doubles * sizeof(double) + 255) & ~(size_t)255;

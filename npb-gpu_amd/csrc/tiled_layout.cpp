@@ -94,7 +94,9 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
     if (m.stored > (long long)INT32_MAX) return false;
     const int nslices = m.wg_slice[nwg];
     m.slice_off.assign(nslices + 1, 0);
+    if ((long long)nslices * kSliceRows > (long long)INT32_MAX) return false;
     m.slice_row.assign((size_t)nslices * kSliceRows, -1);
+    m.row_slot.assign((size_t)m.nblk * n, -1);
     m.val.assign((size_t)m.stored, V(0));
     m.col.assign((size_t)m.stored, 0);
 
@@ -111,6 +113,7 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
             for (int l = 0; l < kSliceRows && i0 + l < segs.size(); l++) {
                 const RowSeg &rs = segs[i0 + l];
                 m.slice_row[(size_t)s * kSliceRows + l] = rs.row;
+                m.row_slot[(size_t)b * n + rs.row] = s * kSliceRows + l;
                 for (int j = 0; j < rs.hi - rs.lo; j++) {
                     const size_t at = (size_t)off + (size_t)j * kSliceRows + l;
                     m.val[at] = (V)a[rs.lo + j];

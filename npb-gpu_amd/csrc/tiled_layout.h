@@ -13,7 +13,12 @@
 //   wg_slice[wg] .. wg_slice[wg+1]   slices of workgroup wg
 //   slice_off[s] .. slice_off[s+1]   entries of slice s (multiple of 64)
 //   slice_row[64*s + l]              row (global) of lane l, -1 for padding
-// Within a row the entries keep their CSR (ascending column) order.
+//   row_slot[b*n + j]                slot 64*s + l of row j in column block b
+// Within a row the entries keep their CSR (ascending column) order.  Every
+// row has one slot in each column block (rows with no entry there sit in a
+// slice with value 0), so the FP64 kernels store a row's block sum at its
+// slot, 64 consecutive sums per slice, and the block sums of row j are found
+// through row_slot.
 #ifndef NPBCG_TILED_LAYOUT_H
 #define NPBCG_TILED_LAYOUT_H
 
@@ -32,6 +37,7 @@ struct SellBlocks {
     std::vector<int> wg_slice;   // nblk * nrange + 1
     std::vector<int> slice_off;  // slices + 1
     std::vector<int> slice_row;  // slices * 64
+    std::vector<int> row_slot;   // nblk * n
     std::vector<V> val;          // stored
     std::vector<uint16_t> col;   // stored
 };
