@@ -493,6 +493,7 @@ def spmv_variant_sweep(reps: int = 20):
                              "gbs_algorithmic": round(row["algorithmic_bytes"] / ms / 1e6, 1)}
                 if v == 5:
                     row["sell_kernels"] = spmv_kernel_rooflines(ctx, m, reps)
+            print(f"spmv sweep {wl['workload']}: {name} {row[name]}", file=sys.stderr, flush=True)
         out[wl["workload"]] = row
     return out
 
@@ -515,6 +516,7 @@ def batch_layout_sweep(reps: int = 10):
                              lanes_per_row=lpr, rows_per_tile=rpt) as bt:
                 rows.append({"layout": layout, "lanes_per_row": lpr, "rows_per_tile": rpt,
                              "spmm_ms": round(bt.spmm_bench(reps), 4)})
+            print(f"batch sweep {name}: {rows[-1]}", file=sys.stderr, flush=True)
         out[name] = rows
     return out
 
