@@ -7,13 +7,14 @@ from plain CG, cg.cu:689-772).
 
 Every system's pair loss is taken before its own Adam steps, so on data the
 network has not seen, relative to the loss of the network's base g = h = 0 (the
-Neumann form).  The shipped run's record reads 1.58 over the first 200 systems
-(an untrained readout) and 0.78 over the last 200.  The test asserts that shape
-and that the freshly trained network cuts the pair loss on held-out systems as
-the shipped one does (mean 0.768 there).  It does not assert bit equality with
-the shipped file: the run is bit-reproducible on one build, but its 4000 Adam
-steps amplify last-bit differences in the logged pairs, which any change to the
-CG kernels' reduction grouping produces."""
+Neumann form).  The shipped run's record: mean ratio 1.64 over the first 200
+systems (the untrained readout), medians 0.836, 0.817, 0.805 and 0.798 over the
+four quarters, median 0.794 over the last 200 (a few systems reach ratios of 2
+to 5 at any stage, so medians carry the trend), and a held-out mean of 0.83.
+The test asserts that shape.  It does not assert bit equality with the shipped
+file: the run is bit-reproducible on one build, but its 4000 Adam steps amplify
+last-bit differences in the logged pairs, which any change to the CG kernels'
+reduction grouping produces."""
 import time
 
 import numpy as np
@@ -45,13 +46,17 @@ def test_full_scale_training_run(gpu):
         st, lh, bh = net.train(o)
         wall = time.perf_counter() - t0
         ratio = lh / bh
-        first, last = float(np.nanmean(ratio[:200])), float(np.nanmean(ratio[-200:]))
+        first_mean = float(np.mean(ratio[:200]))
+        quarters = [float(np.median(ratio[i:i + 500])) for i in range(0, 2000, 500)]
+        last = float(np.median(ratio[-200:]))
         print(f"2000 systems in {wall:.1f} s (generator wait {st.wait_ms / 1e3:.1f} s): "
-              f"loss/base first 200 {first:.3f}, last 200 {last:.3f}, skipped {st.skipped}")
+              f"loss/base mean of the first 200 {first_mean:.3f}, median by quarter "
+              f"{' '.join(f'{q:.3f}' for q in quarters)}, median of the last 200 {last:.3f}, "
+              f"skipped {st.skipped}")
         assert st.systems == 2000 and st.skipped == 0
         assert st.steps == 2000 * o.steps_per_system
         assert np.all(np.isfinite(lh)) and np.all(bh > 0)
-        assert first > 1.0 and last < 0.85
+        assert first_mean > 1.0 and last < 0.85 and quarters[-1] < quarters[0]
         held = []
         for n, nz, shift, seed in _held_out():
             m = gpu.makea(n, nz, shift, seed)
