@@ -39,6 +39,7 @@ struct SellHost {
     std::vector<int> nrange;            // windows per system
     std::vector<long long> part;        // per system: first row partial (elements)
     long long part_total = 0;
+    std::vector<int> off;               // [slices + 1] first entry of each slice
 };
 
 struct npbcg_batch {
@@ -47,6 +48,7 @@ struct npbcg_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     npbcg_batch_opts opts{};
     int nsys = 0, nr = 1, rows = 0;
+    int cus = 256;                   // compute units of the device
     long long nnz = 0;
     std::vector<int> n, row_off;
     std::vector<long long> sys_nnz;
@@ -74,7 +76,7 @@ struct npbcg_batch {
     void *d_sell_val = nullptr;
     uint16_t *d_sell_col = nullptr;
     int *d_sell_off = nullptr, *d_sell_row = nullptr;
-    int *d_block_item = nullptr;     // workgroup -> item of the SELL SpMM
+    int *d_block_item = nullptr;     // workgroup lists of the SELL SpMM (sell_wg_lists)
     size_t block_item_cap = 0;
     // FP64 SELL copy of the true residual: in FP64 mode the inner copy itself
     // (aliased), in MIXED mode a copy of its own with blocks sized for FP64
@@ -90,7 +92,7 @@ struct npbcg_batch {
     int *d_res_block_item = nullptr;
     int res_items = 0, res_item_blocks = 0;
     double *d_res_rows_part = nullptr;
-    int *d_flags = nullptr;          // done | done_next
+    int *d_flags = nullptr;          // done
     BatchScalars *d_sc = nullptr;
     double *d_b = nullptr, *d_x = nullptr, *d_res = nullptr;   // [rows][nr]
     void *d_inner = nullptr;         // d, r, p, q in the inner type
@@ -202,6 +204,73 @@ void make_sell_items(const SellHost &h, int nsys, int g, std::vector<int> *items
     (*sys_item)[nsys] = (int)(items->size() / 4);
 }
 
+// Workgroup lists of the SELL SpMM.  The launch has one workgroup per CU
+// (each holds a 128 KB vector block), kXcds * per_xcd in all; workgroup g
+// walks the items first[g] .. first[g + 1] - 1 of the list (backwards when
+// the launch reverses its order), so no workgroup is dispatched per item and
+// no dispatch slot is left empty.  With at least kXcds systems, whole
+// systems go to XCDs (greedy by cost, largest first; workgroup g runs on XCD
+// g % kXcds), so a system's vector blocks are read through one XCD's L2.
+// Inside an XCD (over all workgroups with fewer systems) the items go,
+// heaviest first, to the least loaded workgroup (longest processing time
+// first: on the 64-system benchmark batch the most loaded workgroup carries
+// 1.5 % more than the mean, where placing system by system left 25 %).  An
+// item costs its stored entries (value and 16-bit column), its vector block
+// and a fixed 16 KB for the fill and the start of its pipeline.
+// *out = first[0 .. G] | items.
+void sell_wg_lists(const npbcg_batch *b, const SellHost &h, const std::vector<int> &items,
+                   const std::vector<int> &sys_item, int elem, std::vector<int> *out) {
+    const int nsys = b->nsys, per_xcd = std::max(1, b->cus / kXcds), G = kXcds * per_xcd;
+    const int ni = sys_item[nsys];
+    std::vector<double> cost(ni), sys_cost(nsys, 0.0);
+    for (int s = 0; s < nsys; s++)
+        for (int it = sys_item[s]; it < sys_item[s + 1]; it++) {
+            const int blk = items[4 * it + 1], sl0 = items[4 * it + 2], sl1 = items[4 * it + 3];
+            const double stored = (double)h.off[sl1] - (double)h.off[sl0];
+            const int w = std::min(h.W, b->n[s] - blk * h.W);
+            cost[it] = stored * (elem + 2.0) + (double)w * b->nr * elem + 16384.0;
+            sys_cost[s] += cost[it];
+        }
+    std::vector<int> order(nsys);
+    for (int s = 0; s < nsys; s++) order[s] = s;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return sys_cost[x] > sys_cost[y]; });
+    // items of each XCD (one pool over all workgroups with fewer systems)
+    const int pools = nsys >= kXcds ? kXcds : 1;
+    std::vector<std::vector<int>> pool(pools);
+    std::vector<double> xload(kXcds, 0.0);
+    for (int s : order) {
+        int x = 0;
+        if (pools > 1) {
+            x = (int)(std::min_element(xload.begin(), xload.end()) - xload.begin());
+            xload[x] += sys_cost[s];
+        }
+        for (int it = sys_item[s]; it < sys_item[s + 1]; it++) pool[x].push_back(it);
+    }
+    std::vector<std::vector<int>> lists(G);
+    std::vector<double> load(G, 0.0);
+    const int stride = pools > 1 ? kXcds : 1, count = pools > 1 ? per_xcd : G;
+    for (int x = 0; x < pools; x++) {
+        std::vector<int> &its = pool[x];
+        std::stable_sort(its.begin(), its.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+        for (int it : its) {
+            int best = x;
+            for (int j = 1; j < count; j++) {
+                const int g = x + j * stride;
+                if (load[g] < load[best]) best = g;
+            }
+            load[best] += cost[it];
+            lists[best].push_back(it);
+        }
+    }
+    out->assign((size_t)G + 1, 0);
+    out->reserve((size_t)G + 1 + ni);
+    for (int g = 0; g < G; g++) {
+        (*out)[g + 1] = (*out)[g] + (int)lists[g].size();
+        out->insert(out->end(), lists[g].begin(), lists[g].end());
+    }
+}
+
 int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
     if (layout == 3 && !b->sell_ok)
         return set_error(NPBCG_ERR_CAPACITY, "batch: no SELL copy (more than 2^31 stored entries)");
@@ -219,11 +288,13 @@ int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
         B_HIP(hipMemcpyAsync(b->d_sysinfo, sys_item.data(), sizeof(int) * (b->nsys + 1),
                              hipMemcpyHostToDevice, b->stream));
         B_HIP(hipStreamSynchronize(b->stream));
-        const std::vector<int> bi = xcd_order(b, sys_item, b->sys_nnz);
-        int rc = upload_order(b, bi, &b->d_block_item, &b->block_item_cap);
+        std::vector<int> wl;
+        sell_wg_lists(b, b->sell_h, items, sys_item,
+                      b->opts.mode == NPBCG_SOLVE_MIXED ? 4 : 8, &wl);
+        int rc = upload_order(b, wl, &b->d_block_item, &b->block_item_cap);
         if (rc != NPBCG_OK) return rc;
         b->bd.items = ni;
-        b->bd.item_blocks = (int)bi.size();
+        b->bd.item_blocks = (int)wl.size() - ni - 1;
         b->bd.block_item = b->d_block_item;
     }
     if (layout == 2) {  // items: (system, column block, row range) in system order
@@ -338,6 +409,7 @@ bool build_batch_sell(const npbcg_batch *b, int W, const std::vector<int> &rp,
         slices += nsl;
     }
     soff->push_back((int)entries);
+    h->off = *soff;
     return true;
 }
 
@@ -583,6 +655,7 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
                                                         hipGetErrorString(_e))); \
     } while (0)
     TRY(hipSetDevice(b->device));
+    TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device));
     TRY(hipStreamCreate(&b->stream));
     TRY(hipEventCreateWithFlags(&b->ev0, hipEventDisableSystemFence));
     TRY(hipEventCreateWithFlags(&b->ev1, hipEventDisableSystemFence));
@@ -672,7 +745,7 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     if (o.mode == NPBCG_SOLVE_MIXED)
         TRY(hipMalloc(&b->d_val32, sizeof(float) * std::max<long long>(nnz, 1)));
     TRY(hipMalloc(&b->d_tiles, sizeof(int) * (2 * (size_t)b->tiles_cap + nsys + 2)));
-    TRY(hipMalloc(&b->d_flags, sizeof(int) * 2 * nsys * b->nr));
+    TRY(hipMalloc(&b->d_flags, sizeof(int) * nsys * b->nr));
     TRY(hipMalloc(&b->d_sc, sizeof(BatchScalars) * nsys * b->nr));
     TRY(hipMalloc(&b->d_b, sizeof(double) * m));
     TRY(hipMalloc(&b->d_x, sizeof(double) * m));
@@ -743,8 +816,9 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
             sys[nsys + 1 + s] = b->row_off[s];
         }
         for (int s = 0; s < nsys; s++) sys[2 * nsys + 2 + s] = b->res_h.nb[s];
-        const std::vector<int> order = xcd_order(b, sys_item, b->sys_nnz);
-        b->res_item_blocks = (int)order.size();
+        std::vector<int> order;
+        sell_wg_lists(b, b->res_h, items, sys_item, 8, &order);
+        b->res_item_blocks = (int)order.size() - b->res_items - 1;
         TRY(hipMalloc(&b->d_res_items, sizeof(int) * std::max<size_t>(items.size(), 4)));
         TRY(hipMalloc(&b->d_res_sys, sizeof(int) * sys.size()));
         TRY(hipMalloc(&b->d_res_part, sizeof(long long) * nsys));
@@ -771,7 +845,7 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
                            b->stream));
     TRY(hipMemsetAsync(b->d_x, 0, sizeof(double) * m, b->stream));
     TRY(hipMemsetAsync(b->d_inner, 0, inner_elem * 4 * m, b->stream));
-    TRY(hipMemsetAsync(b->d_flags, 0, sizeof(int) * 2 * nsys * b->nr, b->stream));
+    TRY(hipMemsetAsync(b->d_flags, 0, sizeof(int) * nsys * b->nr, b->stream));
     TRY(hipMemsetAsync(b->d_sc, 0, sizeof(BatchScalars) * nsys * b->nr, b->stream));
     TRY(hipStreamSynchronize(b->stream));  // host staging goes out of scope
 #undef TRY
@@ -781,7 +855,6 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     b->bd.rowptr = b->d_rowptr;
     b->bd.col = b->d_col;
     b->bd.done = b->d_flags;
-    b->bd.done_next = b->d_flags + nsys * b->nr;
     b->bd.sc = b->d_sc;
     b->bd.cbw = b->cbw;
     b->bd.item = b->d_items;
@@ -840,7 +913,7 @@ int npbcg_batch_tune(npbcg_batch *b, const char *cache_path, int *lanes_per_row,
     if (!cache_lookup(cache_path, key, &lpr, &rt, &layout, &best)) {
         // All solves marked running so every tile does its work; vectors hold
         // whatever they hold (timing only); state is reset afterwards.
-        B_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * 2 * b->nsys * b->nr, b->stream));
+        B_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * b->nsys * b->nr, b->stream));
         best = 1e30;
         int dl = 0, dr = 0;
         default_shape(b, &dl, &dr);
@@ -925,8 +998,6 @@ int npbcg_batch_solve(npbcg_batch *b, double rtol, int maxit, npbcg_batch_stats 
                              b->stream));
         B_HIP(hipMemcpyAsync(b->bd.done, done.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
                              b->stream));
-        B_HIP(hipMemcpyAsync(b->bd.done_next, done.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
-                             b->stream));
         int rc = mixed ? run_inner<float>(b, b->bd.maxit) : run_inner<double>(b, b->bd.maxit);
         if (rc != NPBCG_OK) return rc;
         rc = true_residual(b);
@@ -966,7 +1037,7 @@ int npbcg_batch_solve(npbcg_batch *b, double rtol, int maxit, npbcg_batch_stats 
 int npbcg_batch_spmm_bench(npbcg_batch *b, int reps, double *avg_ms) {
     if (!b || reps < 1 || !avg_ms) return set_error(NPBCG_ERR_INVALID, "npbcg_batch_spmm_bench: invalid argument");
     B_HIP(hipSetDevice(b->device));
-    B_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * 2 * b->nsys * b->nr, b->stream));
+    B_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * b->nsys * b->nr, b->stream));
     int rc = b->opts.mode == NPBCG_SOLVE_MIXED ? time_spmm<float>(b, reps, avg_ms)
                                                : time_spmm<double>(b, reps, avg_ms);
     if (rc != NPBCG_OK) return rc;

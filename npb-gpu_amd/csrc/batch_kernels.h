@@ -61,13 +61,13 @@ struct BatchDev {
     // columns inside the block).  Item = (system, block, first slice, end
     // slice); the row partials and p.q partials are laid out as for cb.
     int sell = 0;
-    int item_blocks = 0;               // workgroups of the SELL SpMM
-    const int *block_item = nullptr;   // [item_blocks] item of each workgroup, -1: none
+    int item_blocks = 0;               // workgroups of the SELL SpMM (one per CU)
+    const int *block_item = nullptr;   // [item_blocks + 1] list offsets, then the lists:
+                                       // workgroup g runs items list[first[g] .. first[g+1])
     const int *sell_off = nullptr;     // [slices + 1] first entry of each slice
     const int *sell_row = nullptr;     // [slices * 64] global row of each lane, -1: padding
     const uint16_t *sell_col = nullptr;  // [stored] column inside the block
     int *done = nullptr;            // [nsys * nr] right-hand side finished
-    int *done_next = nullptr;       // flags set during a step, published after it
     BatchScalars *sc = nullptr;     // [nsys * nr]
 };
 
@@ -82,7 +82,7 @@ struct BatchVecs {
 };
 
 // Step `step` of an inner solve of every unfinished system: SpMM + p.q,
-// x/r update + r.r, p update + stop test, and the flag publish.
+// x/r update + r.r, p update + stop test.
 template <typename T>
 hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, int first,
                          hipStream_t st);

@@ -7,9 +7,9 @@
 // rows that never cross a system boundary, one workgroup per tile, so every
 // per-system reduction is a fixed-order sum over that system's tile partials
 // and no atomics are needed.  One CG step for the whole batch is three
-// launches (SpMM + p.q, x/r update + r.r, p update + stop test) and a flag
-// publish; a system whose right-hand sides have all converged costs its
-// workgroups one flag read.
+// launches (SpMM + p.q, x/r update + r.r, p update + stop test); a system
+// whose right-hand sides have all converged costs its workgroups one flag
+// read.
 //
 // The SpMM reads each matrix entry once for all NR vectors.  Three layouts:
 // row gathers (LPR lanes share a CSR row, every lane gathers the NR values of
@@ -288,12 +288,12 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_cb_kernel(BatchDev bd,
 // The entry stream is software-pipelined with sell_chunk: the loads of the
 // next chunk (or of the next slice's first chunk and its lane rows) are issued
 // before the current chunk's LDS gathers and FMAs, and the first chunk is
-// issued before the LDS fill.
+// issued before the LDS fill, whose loads all go out before its LDS stores.
 //
-// rev = 1 takes each XCD's items in reverse order (workgroup 8 i + x runs the
-// item listed at 8 (L - 1 - i) + x, L = item_blocks / 8): successive SpMMs
-// alternate, so each starts on the items the last one streamed last, which
-// the memory-side cache still holds.  Systems keep their XCD.
+// One workgroup per CU walks its own list of items (block_item: item_blocks
+// + 1 list offsets, then the lists), with a barrier between items.  rev = 1
+// walks the list backwards: successive SpMMs alternate, so each starts on the
+// items the last one streamed last, which the memory-side cache still holds.
 template <typename T, int NR, int BLOCK, int UNR, bool RESID>
 __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
                                                                 const T *__restrict__ sval,
@@ -304,12 +304,12 @@ __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
     extern __shared__ double2 lds_sell[];  // 16-byte aligned
     T *vlds = reinterpret_cast<T *>(lds_sell);
     constexpr int kWaves = BLOCK / kWave;
-    const int slot = rev ? bd.item_blocks - 8 - 8 * (int)(blockIdx.x / 8) + (int)(blockIdx.x % 8)
-                         : (int)blockIdx.x;
-    const int it = bd.block_item[slot];
-    if (it < 0) return;
+    const int *__restrict__ wl = bd.block_item + bd.item_blocks + 1;
+    const int k0 = bd.block_item[blockIdx.x], k1 = bd.block_item[blockIdx.x + 1];
+    for (int li = k0; li < k1; li++) {
+    const int it = wl[rev ? k0 + k1 - 1 - li : li];
     const int s = bd.item[4 * it];
-    if (!RESID && system_done<NR>(bd.done, s)) return;
+    if (!RESID && system_done<NR>(bd.done, s)) continue;  // the same for every thread
     const int b = bd.item[4 * it + 1], sl0 = bd.item[4 * it + 2];
     const int nsl = bd.item[4 * it + 3] - sl0;
     const int srow = bd.sys_row[s], ns = bd.sys_row[s + 1] - srow;
@@ -338,10 +338,13 @@ __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
         nrow = lane_row[(size_t)cur * kWave + lane];
         if (cwidth > 0) sell_chunk<T, UNR>(sval, col, cbase, cwidth, 0, lane, na, nc);
     }
-    {
-        const T *src = p + (size_t)(srow + c0) * NR;
-        for (int i = threadIdx.x; i < w; i += BLOCK)
-            store_vec<T, NR>(vlds + i * NR, load_vec<T, NR>(src + (size_t)i * NR));
+    if constexpr (NR * sizeof(T) % 16 == 0) {  // rows of whole 16-byte units, 16-byte aligned
+        constexpr int kUnits = NR * (int)sizeof(T) / 16;
+        stage_block<BLOCK, (128 * 1024 / 16 + BLOCK - 1) / BLOCK>(
+            reinterpret_cast<const double2 *>(p + (size_t)(srow + c0) * NR), lds_sell, w * kUnits);
+    } else {
+        stage_block<BLOCK, (128 * 1024 / (int)sizeof(T) + BLOCK - 1) / BLOCK>(
+            p + (size_t)(srow + c0) * NR, vlds, w * NR);
     }
     __syncthreads();
     T *part = rows_part + bd.sys_part[s] + (size_t)b * ns * NR;
@@ -403,6 +406,8 @@ __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
         cwidth = xwidth;
     }
     if constexpr (!RESID) block_sum_store<NR, BLOCK>(dot, pq_part + (size_t)it * NR);
+    __syncthreads();  // the vector block and the reduction slots are free again
+    }
 }
 
 // FP64 true residual from the SELL row partials of A x: res = b - sum over
@@ -539,7 +544,11 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(BatchDev bd, const
 }
 
 // beta = r.r / rho per RHS; stop test r.r <= tol2 (the tile's system's first
-// tile records it); p = r + beta p for the right-hand sides still running.
+// tile sets the flag); p = r + beta p for the right-hand sides still running.
+// The flag is written during the launch: another tile of the system that
+// reads it already set takes the right-hand side as inactive, which skips the
+// same p update the stop test skips, so every tile does the same work either
+// way and the next step's kernels see the flag.
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void batch_pupdate_kernel(BatchDev bd,
                                                                const T *__restrict__ r,
@@ -566,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void batch_pupdate_kernel(BatchDev bd,
     if (t == t0)
 #pragma unroll
         for (int k = 0; k < NR; k++)
-            if (threadIdx.x == k && bd.done[s * NR + k] == 0 && !run[k]) bd.done_next[s * NR + k] = 1;
+            if (threadIdx.x == k && bd.done[s * NR + k] == 0 && !run[k]) bd.done[s * NR + k] = 1;
     const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
     for (int row = row0 + threadIdx.x; row < row1; row += kBlock)
 #pragma unroll
@@ -575,14 +584,6 @@ __global__ __launch_bounds__(kBlock) void batch_pupdate_kernel(BatchDev bd,
                 const size_t at = (size_t)row * NR + k;
                 p[at] = fma(beta[k], p[at], r[at]);
             }
-}
-
-// done = done_next (the stop flags of the step just finished become visible
-// to the next step's kernels, never to the current step's other workgroups).
-__global__ void batch_publish_kernel(int *__restrict__ done, const int *__restrict__ done_next,
-                                     int count) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock)
-        done[i] = done_next[i];
 }
 
 // Start of an inner solve: r = p = (T)(b - A x) (given in res), d = 0,
@@ -764,10 +765,7 @@ hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, in
         batch_pupdate_kernel<T, NR><<<tiles, kBlock, 0, st>>>(bd, v.r, v.p, rr, rho);
         return hipGetLastError();
     });
-    if (e != hipSuccess) return e;
-    batch_publish_kernel<<<grid_for(bd.nsys * bd.nr), kBlock, 0, st>>>(bd.done, bd.done_next,
-                                                                       bd.nsys * bd.nr);
-    return hipGetLastError();
+    return e;
 }
 
 template <typename T>
