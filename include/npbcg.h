@@ -406,8 +406,9 @@ typedef struct npbcg_batch_opts {
     int mode;            /* NPBCG_SOLVE_FP64 or NPBCG_SOLVE_MIXED */
     int lanes_per_row;   /* lanes per row of the row-gather kernels: 4, 8, 16, 32, 64;
                             0 = tuned or default */
-    int rows_per_tile;   /* rows per SpMM workgroup: 64 .. 4096 (layout 3: rounded
-                            down to a multiple of 2048); 0 = tuned or default */
+    int rows_per_tile;   /* rows per SpMM workgroup: 64 .. 4096 (layout 3: one
+                            2048-row sorting window per item, whatever is given);
+                            0 = tuned or default */
     int inner_steps;     /* MIXED: inner CG steps per sweep at most (0: 60) */
     int layout;          /* SpMM: 0 = tuned or the default (3), 1 = row gathers,
                             2 = column blocks (vector blocks staged in LDS, rows

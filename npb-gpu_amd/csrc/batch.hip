@@ -78,6 +78,8 @@ struct npbcg_batch {
     int *d_sell_off = nullptr, *d_sell_row = nullptr;
     int *d_block_item = nullptr;     // workgroup lists of the SELL SpMM (sell_wg_lists)
     size_t block_item_cap = 0;
+    int *d_item_sl = nullptr;        // slice ranges of the SELL items (make_sell_items)
+    size_t item_sl_cap = 0;
     // FP64 SELL copy of the true residual: in FP64 mode the inner copy itself
     // (aliased), in MIXED mode a copy of its own with blocks sized for FP64
     // vectors.  Items are one sorting window each.
@@ -90,8 +92,8 @@ struct npbcg_batch {
     int *d_res_sys = nullptr;        // sys_item | sys_row | sys_nb
     long long *d_res_part = nullptr; // sys_part
     int *d_res_block_item = nullptr;
+    int *d_res_item_sl = nullptr;
     int res_items = 0, res_item_blocks = 0;
-    double *d_res_rows_part = nullptr;
     int *d_flags = nullptr;          // done
     BatchScalars *d_sc = nullptr;
     double *d_b = nullptr, *d_x = nullptr, *d_res = nullptr;   // [rows][nr]
@@ -183,29 +185,36 @@ int upload_order(npbcg_batch *b, const std::vector<int> &order, int **buf, size_
 // Rows per tile of the vector kernels under the SELL layout.
 constexpr int kSellTileRows = 256;
 
-// SELL items (system, block, first slice, end slice) of g windows each, in
-// system order, and the first item of each system.
-void make_sell_items(const SellHost &h, int nsys, int g, std::vector<int> *items,
-                     std::vector<int> *sys_item) {
+// SELL items, one per sorting window: (system, first local row, rows, offset
+// into *item_sl), in system order, and the first item of each system.
+// item_sl holds, for each item, the window's slice range [first, end) in
+// each column block of its system.
+void make_sell_items(const SellHost &h, const std::vector<int> &n, std::vector<int> *items,
+                     std::vector<int> *sys_item, std::vector<int> *item_sl) {
+    const int nsys = (int)n.size();
     items->clear();
+    item_sl->clear();
     sys_item->assign(nsys + 1, 0);
     for (int s = 0; s < nsys; s++) {
         (*sys_item)[s] = (int)(items->size() / 4);
         const int nw = h.nrange[s];
         const std::vector<int> &wg = h.wg[s];
-        for (int blk = 0; blk < h.nb[s]; blk++)
-            for (int r = 0; r < nw; r += g) {
-                items->push_back(s);
-                items->push_back(blk);
-                items->push_back(wg[(size_t)blk * nw + r]);
-                items->push_back(wg[(size_t)blk * nw + std::min(nw, r + g)]);
+        for (int r = 0; r < nw; r++) {
+            items->push_back(s);
+            items->push_back(r * kSellWindow);
+            items->push_back(std::min(kSellWindow, n[s] - r * kSellWindow));
+            items->push_back((int)item_sl->size());
+            for (int blk = 0; blk < h.nb[s]; blk++) {
+                item_sl->push_back(wg[(size_t)blk * nw + r]);
+                item_sl->push_back(wg[(size_t)blk * nw + r + 1]);
             }
+        }
     }
     (*sys_item)[nsys] = (int)(items->size() / 4);
 }
 
 // Workgroup lists of the SELL SpMM.  The launch has one workgroup per CU
-// (each holds a 128 KB vector block), kXcds * per_xcd in all; workgroup g
+// (each holds a vector block and a window's row sums), kXcds * per_xcd in all; workgroup g
 // walks the items first[g] .. first[g + 1] - 1 of the list (backwards when
 // the launch reverses its order), so no workgroup is dispatched per item and
 // no dispatch slot is left empty.  With at least kXcds systems, whole
@@ -213,22 +222,24 @@ void make_sell_items(const SellHost &h, int nsys, int g, std::vector<int> *items
 // g % kXcds), so a system's vector blocks are read through one XCD's L2.
 // Inside an XCD (over all workgroups with fewer systems) the items go,
 // heaviest first, to the least loaded workgroup (longest processing time
-// first: on the 64-system benchmark batch the most loaded workgroup carries
-// 1.5 % more than the mean, where placing system by system left 25 %).  An
-// item costs its stored entries (value and 16-bit column), its vector block
-// and a fixed 16 KB for the fill and the start of its pipeline.
+// first).  An item costs its stored entries (value and 16-bit column), the
+// vector blocks it stages, its rows' q and p, and a fixed 16 KB per block for
+// the fill and the start of the block's pipeline.
 // *out = first[0 .. G] | items.
 void sell_wg_lists(const npbcg_batch *b, const SellHost &h, const std::vector<int> &items,
-                   const std::vector<int> &sys_item, int elem, std::vector<int> *out) {
+                   const std::vector<int> &sys_item, const std::vector<int> &item_sl, int elem,
+                   std::vector<int> *out) {
     const int nsys = b->nsys, per_xcd = std::max(1, b->cus / kXcds), G = kXcds * per_xcd;
     const int ni = sys_item[nsys];
     std::vector<double> cost(ni), sys_cost(nsys, 0.0);
     for (int s = 0; s < nsys; s++)
         for (int it = sys_item[s]; it < sys_item[s + 1]; it++) {
-            const int blk = items[4 * it + 1], sl0 = items[4 * it + 2], sl1 = items[4 * it + 3];
-            const double stored = (double)h.off[sl1] - (double)h.off[sl0];
-            const int w = std::min(h.W, b->n[s] - blk * h.W);
-            cost[it] = stored * (elem + 2.0) + (double)w * b->nr * elem + 16384.0;
+            const int nrows = items[4 * it + 2], so = items[4 * it + 3];
+            double stored = 0.0;
+            for (int blk = 0; blk < h.nb[s]; blk++)
+                stored += (double)h.off[item_sl[so + 2 * blk + 1]] - (double)h.off[item_sl[so + 2 * blk]];
+            cost[it] = stored * (elem + 2.0) + (double)(b->n[s] + 2 * nrows) * b->nr * elem +
+                       16384.0 * h.nb[s];
             sys_cost[s] += cost[it];
         }
     std::vector<int> order(nsys);
@@ -276,11 +287,10 @@ int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
         return set_error(NPBCG_ERR_CAPACITY, "batch: no SELL copy (more than 2^31 stored entries)");
     std::vector<int> ts, tr, st;
     make_tiles(b, layout == 3 ? kSellTileRows : rt, ts, tr, st);
-    if (layout == 3) {  // items: (system, block, slices of g windows) in system order
-        const int g = std::max(1, rt / kSellWindow);
-        rt = g * kSellWindow;
-        std::vector<int> items, sys_item;
-        make_sell_items(b->sell_h, b->nsys, g, &items, &sys_item);
+    if (layout == 3) {  // items: one sorting window each, in system order
+        rt = kSellWindow;
+        std::vector<int> items, sys_item, item_sl;
+        make_sell_items(b->sell_h, b->n, &items, &sys_item, &item_sl);
         const int ni = sys_item[b->nsys];
         if (ni > b->items_cap) return set_error(NPBCG_ERR_INTERNAL, "batch: item capacity");
         B_HIP(hipMemcpyAsync(b->d_items, items.data(), sizeof(int) * items.size(),
@@ -288,14 +298,17 @@ int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
         B_HIP(hipMemcpyAsync(b->d_sysinfo, sys_item.data(), sizeof(int) * (b->nsys + 1),
                              hipMemcpyHostToDevice, b->stream));
         B_HIP(hipStreamSynchronize(b->stream));
+        int rc = upload_order(b, item_sl, &b->d_item_sl, &b->item_sl_cap);
+        if (rc != NPBCG_OK) return rc;
         std::vector<int> wl;
-        sell_wg_lists(b, b->sell_h, items, sys_item,
+        sell_wg_lists(b, b->sell_h, items, sys_item, item_sl,
                       b->opts.mode == NPBCG_SOLVE_MIXED ? 4 : 8, &wl);
-        int rc = upload_order(b, wl, &b->d_block_item, &b->block_item_cap);
+        rc = upload_order(b, wl, &b->d_block_item, &b->block_item_cap);
         if (rc != NPBCG_OK) return rc;
         b->bd.items = ni;
         b->bd.item_blocks = (int)wl.size() - ni - 1;
         b->bd.block_item = b->d_block_item;
+        b->bd.item_sl = b->d_item_sl;
     }
     if (layout == 2) {  // items: (system, column block, row range) in system order
         std::vector<int> items, sys_item(b->nsys + 1, 0);
@@ -347,19 +360,23 @@ int apply_shape(npbcg_batch *b, int lpr, int rt, int layout) {
     return NPBCG_OK;
 }
 
-// Sizes of the column-blocked layouts for the batch's inner type: blocks of
-// cbw columns whose nrhs-wide vector entries fill 128 KB of LDS (one SELL
-// workgroup per CU).  Wider blocks than half that measured faster for the
-// SELL SpMM: half as many row partials, twice as long row segments.
-int cb_width(const npbcg_batch *b, int device) {
+// Column block width of a SELL copy with elem-byte vector entries: the
+// block's nrhs-wide entries and the row sums of one 2048-row window fill the
+// workgroup's LDS (160 KB on gfx950, less 2 KB for the static reduction
+// slots): one SELL workgroup per CU.  Four FP64 right-hand sides get 3008
+// columns, four FP32 ones 8064.
+int sell_width(const npbcg_batch *b, int device, int elem) {
     int optin = 0;
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) !=
             hipSuccess || optin <= 0)
         optin = 64 * 1024;
-    const int elem = b->opts.mode == NPBCG_SOLVE_MIXED ? 4 : 8;
-    const int want = 128 * 1024;
-    const int lds = std::min(want, optin - 2048);
-    return std::max(64, lds / (elem * b->nr));
+    const int lds = optin - 2048;
+    return std::min(65536, std::max(64, lds / (elem * b->nr) - kSellWindow));
+}
+
+// Column blocks of the inner type (layouts 2 and 3 share them).
+int cb_width(const npbcg_batch *b, int device) {
+    return sell_width(b, device, b->opts.mode == NPBCG_SOLVE_MIXED ? 4 : 8);
 }
 
 // SELL copy of every system (local CSR with ascending columns per row, as
@@ -413,16 +430,8 @@ bool build_batch_sell(const npbcg_batch *b, int W, const std::vector<int> &rp,
     return true;
 }
 
-// Column block width of the FP64 residual copy: nrhs-wide FP64 vector
-// entries in 128 KB of LDS.
-int res_width(const npbcg_batch *b, int device) {
-    int optin = 0;
-    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) !=
-            hipSuccess || optin <= 0)
-        optin = 64 * 1024;
-    const int lds = std::min(128 * 1024, optin - 2048);
-    return std::max(64, lds / (8 * b->nr));
-}
+// Column block width of the FP64 residual copy.
+int res_width(const npbcg_batch *b, int device) { return sell_width(b, device, 8); }
 
 template <typename T>
 BatchVecs<T> inner_vecs(const npbcg_batch *b) {
@@ -464,14 +473,15 @@ BatchDev resid_dev(const npbcg_batch *b) {
     d.sell_off = b->d_res_off;
     d.sell_row = b->d_res_row;
     d.sell_col = b->d_res_col;
+    d.item_sl = b->d_res_item_sl;
     return d;
 }
 
 // res = b - A x (FP64), |res|^2 per solve into d_sysout.
 int true_residual(npbcg_batch *b) {
     if (b->res_ok)
-        B_HIP(batch_residual_sell(resid_dev(b), b->d_res_val, b->d_b, b->d_x, b->d_res_rows_part,
-                                  b->d_res, b->d_part, b->d_sysout, b->stream));
+        B_HIP(batch_residual_sell(resid_dev(b), b->d_res_val, b->d_b, b->d_x, b->d_res, b->d_part,
+                                  b->d_sysout, b->stream));
     else
         B_HIP(batch_residual(b->bd, b->d_val, b->d_b, b->d_x, b->d_res, b->d_part, b->d_sysout,
                              b->stream));
@@ -480,8 +490,9 @@ int true_residual(npbcg_batch *b) {
 
 std::string shape_key(const npbcg_batch *b) {
     char buf[256];
-    // "batch3": records of the three-layout tuner (older records are not reused)
-    std::snprintf(buf, sizeof buf, "batch3:sys=%d:rows=%d:nnz=%lld:nrhs=%d:mode=%d", b->nsys,
+    // "batch4": records of the tuner with window items for layout 3 (older
+    // records are not reused)
+    std::snprintf(buf, sizeof buf, "batch4:sys=%d:rows=%d:nnz=%lld:nrhs=%d:mode=%d", b->nsys,
                   b->rows, b->nnz, b->nr, b->opts.mode);
     return buf;
 }
@@ -578,13 +589,13 @@ void npbcg_batch_destroy(npbcg_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->res_owns)
         for (void *p : {(void *)b->d_res_val, (void *)b->d_res_col, (void *)b->d_res_off,
-                        (void *)b->d_res_row, (void *)b->d_res_rows_part})
+                        (void *)b->d_res_row})
             (void)hipFree(p);
     for (void *p : {(void *)b->d_res_items, (void *)b->d_res_sys, (void *)b->d_res_part,
-                    (void *)b->d_res_block_item})
+                    (void *)b->d_res_block_item, (void *)b->d_res_item_sl})
         (void)hipFree(p);
     for (void *p : {b->d_sell_val, (void *)b->d_sell_col, (void *)b->d_sell_off,
-                    (void *)b->d_sell_row, (void *)b->d_block_item,
+                    (void *)b->d_sell_row, (void *)b->d_block_item, (void *)b->d_item_sl,
                     (void *)b->d_items, (void *)b->d_sysinfo, (void *)b->d_sys64, (void *)b->d_seg,
                     b->d_rows_part, (void *)b->d_rowptr, (void *)b->d_col, (void *)b->d_val,
                     (void *)b->d_val32,
@@ -790,8 +801,6 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
             TRY(hipMalloc(&b->d_res_col, std::max<size_t>(sizeof(uint16_t) * stored, 8)));
             TRY(hipMalloc(&b->d_res_off, sizeof(int) * roff.size()));
             TRY(hipMalloc(&b->d_res_row, sizeof(int) * std::max<size_t>(rrow.size(), 1)));
-            TRY(hipMalloc(&b->d_res_rows_part,
-                          sizeof(double) * std::max<long long>(b->res_h.part_total, 1)));
             TRY(hipMemcpyAsync(b->d_res_val, rval.data(), sizeof(double) * stored,
                                hipMemcpyHostToDevice, b->stream));
             TRY(hipMemcpyAsync(b->d_res_col, rcol.data(), sizeof(uint16_t) * stored,
@@ -800,15 +809,14 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
                                hipMemcpyHostToDevice, b->stream));
             TRY(hipMemcpyAsync(b->d_res_row, rrow.data(), sizeof(int) * rrow.size(),
                                hipMemcpyHostToDevice, b->stream));
-        } else {  // FP64 mode: the inner SELL copy and its row partials
+        } else {  // FP64 mode: the inner SELL copy
             b->d_res_val = static_cast<double *>(b->d_sell_val);
             b->d_res_col = b->d_sell_col;
             b->d_res_off = b->d_sell_off;
             b->d_res_row = b->d_sell_row;
-            b->d_res_rows_part = static_cast<double *>(b->d_rows_part);
         }
-        std::vector<int> items, sys_item;
-        make_sell_items(b->res_h, nsys, 1, &items, &sys_item);
+        std::vector<int> items, sys_item, item_sl;
+        make_sell_items(b->res_h, b->n, &items, &sys_item, &item_sl);
         b->res_items = sys_item[nsys];
         std::vector<int> sys(3 * (size_t)nsys + 2);
         for (int s = 0; s <= nsys; s++) {
@@ -817,12 +825,15 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
         }
         for (int s = 0; s < nsys; s++) sys[2 * nsys + 2 + s] = b->res_h.nb[s];
         std::vector<int> order;
-        sell_wg_lists(b, b->res_h, items, sys_item, 8, &order);
+        sell_wg_lists(b, b->res_h, items, sys_item, item_sl, 8, &order);
         b->res_item_blocks = (int)order.size() - b->res_items - 1;
         TRY(hipMalloc(&b->d_res_items, sizeof(int) * std::max<size_t>(items.size(), 4)));
         TRY(hipMalloc(&b->d_res_sys, sizeof(int) * sys.size()));
         TRY(hipMalloc(&b->d_res_part, sizeof(long long) * nsys));
         TRY(hipMalloc(&b->d_res_block_item, sizeof(int) * std::max<size_t>(order.size(), 1)));
+        TRY(hipMalloc(&b->d_res_item_sl, sizeof(int) * std::max<size_t>(item_sl.size(), 1)));
+        TRY(hipMemcpyAsync(b->d_res_item_sl, item_sl.data(), sizeof(int) * item_sl.size(),
+                           hipMemcpyHostToDevice, b->stream));
         TRY(hipMemcpyAsync(b->d_res_items, items.data(), sizeof(int) * items.size(),
                            hipMemcpyHostToDevice, b->stream));
         TRY(hipMemcpyAsync(b->d_res_sys, sys.data(), sizeof(int) * sys.size(),
@@ -831,7 +842,8 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
                            hipMemcpyHostToDevice, b->stream));
         TRY(hipMemcpyAsync(b->d_res_block_item, order.data(), sizeof(int) * order.size(),
                            hipMemcpyHostToDevice, b->stream));
-        const hipError_t e3 = batch_prepare_resid(b->res_h.W * b->nr * (int)sizeof(double));
+        const hipError_t e3 =
+            batch_prepare_resid((int)sell_lds_bytes(b->res_h.W, b->nr, sizeof(double)));
         if (e3 != hipSuccess)
             return fail(set_error(NPBCG_ERR_DEVICE, std::string("batch_prepare_resid: ") +
                                                         hipGetErrorString(e3)));
@@ -868,7 +880,7 @@ int npbcg_batch_create(npbcg_batch **out, int nsys, const int *n, const int *con
     b->bd.sell_row = b->d_sell_row;
     b->bd.sell_col = b->d_sell_col;
     {
-        const int lds = b->cbw * b->nr * (o.mode == NPBCG_SOLVE_MIXED ? 4 : 8);
+        const int lds = (int)sell_lds_bytes(b->cbw, b->nr, o.mode == NPBCG_SOLVE_MIXED ? 4 : 8);
         hipError_t e2 = o.mode == NPBCG_SOLVE_MIXED ? batch_prepare_cb<float>(lds)
                                                     : batch_prepare_cb<double>(lds);
         if (e2 != hipSuccess)
@@ -922,10 +934,10 @@ int npbcg_batch_tune(npbcg_batch *b, const char *cache_path, int *lanes_per_row,
         for (int cy : {1, 3})
             for (int cl : {4, 8, 16, 32, 64})
                 for (int cr : {64, 128, 256, 512, 1024, 2048, 4096}) {
-                    // layout 3: one lane per row, items of whole sorting
-                    // windows; lanes_per_row stays at its default (it shapes
+                    // layout 3: one lane per row, one sorting window per
+                    // item; lanes_per_row stays at its default (it shapes
                     // only the row-gather residual kernel)
-                    if (cy == 3 && (!b->sell_ok || cl != dl || cr < kSellWindow)) continue;
+                    if (cy == 3 && (!b->sell_ok || cl != dl || cr != kSellWindow)) continue;
                     int rc = apply_shape(b, cl, cr, cy);
                     if (rc != NPBCG_OK) return rc;
                     double t = 0;

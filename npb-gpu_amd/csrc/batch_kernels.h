@@ -58,9 +58,12 @@ struct BatchDev {
     // SELL SpMM (sell = 1, also cb = 1): a column-blocked SELL copy of every
     // system (blocks of cbw columns, rows sorted by segment length inside
     // kSellWindow-row windows, 64-row slices stored column-major, 16-bit
-    // columns inside the block).  Item = (system, block, first slice, end
-    // slice); the row partials and p.q partials are laid out as for cb.
+    // columns inside the block).  Item = one window: (system, first local row,
+    // rows, offset into item_sl), where item_sl holds the window's slice range
+    // [first, end) in each of the system's blocks.  The SpMM writes q (not
+    // block partials) and one p.q partial per item.
     int sell = 0;
+    const int *item_sl = nullptr;
     int item_blocks = 0;               // workgroups of the SELL SpMM (one per CU)
     const int *block_item = nullptr;   // [item_blocks + 1] list offsets, then the lists:
                                        // workgroup g runs items list[first[g] .. first[g+1])
@@ -86,6 +89,11 @@ struct BatchVecs {
 template <typename T>
 hipError_t batch_cg_step(const BatchDev &bd, const BatchVecs<T> &v, int step, int first,
                          hipStream_t st);
+// Dynamic LDS of the SELL SpMM: the vector block (cbw rows of nr values)
+// and the row sums of one sorting window.
+inline size_t sell_lds_bytes(int cbw, int nr, size_t elem) {
+    return (size_t)(cbw + kSellWindow) * nr * elem;
+}
 // Allows `bytes` of dynamic LDS for the column-blocked and SELL SpMM kernels
 // of type T.
 template <typename T>
@@ -108,12 +116,12 @@ hipError_t batch_refine(const BatchDev &bd, double *x, const T *d, hipStream_t s
 // res = b - A x (FP64) and |res|^2 per (system, rhs) into sys_out[nsys * nr].
 hipError_t batch_residual(const BatchDev &bd, const double *val, const double *b, const double *x,
                           double *res, double *part, double *sys_out, hipStream_t st);
-// The same from an FP64 SELL copy described by bd (its items, column blocks
-// and partial offsets): row partials of A x into rows_part (FP64), then the
-// combine.
+// The same from an FP64 SELL copy described by bd (its window items and
+// column blocks): one SELL SpMM writes res and the |res|^2 partials of its
+// items, then the per-system sums.
 hipError_t batch_residual_sell(const BatchDev &bd, const double *sval, const double *b,
-                               const double *x, double *rows_part, double *res, double *part,
-                               double *sys_out, hipStream_t st);
+                               const double *x, double *res, double *part, double *sys_out,
+                               hipStream_t st);
 // Residual of x = 0 (res = b).
 hipError_t batch_residual_zero(const BatchDev &bd, const double *b, double *res, double *part,
                                double *sys_out, hipStream_t st);

@@ -277,174 +277,172 @@ __global__ __launch_bounds__(kBlock) void batch_spmm_cb_kernel(BatchDev bd,
     block_sum_store<NR>(dot, pq_part + (size_t)it * NR);
 }
 
-// SELL SpMM: item (system s, column block b, slices [sl0, sl1)).  The
-// block's vector entries (cbw rows of p, NR values each) are staged in LDS;
-// each wavefront then streams whole slices, one row per lane, so the (value,
-// 16-bit column) loads are contiguous across the wavefront and a row needs no
-// cross-lane reduction.  The row's block sum goes to the block's row partials
-// (added up by the update kernel); p.q partials follow by linearity.  Slices
-// go to wavefronts in a fixed snake order, so the sums are reproducible.
+// SELL SpMM over one 2048-row sorting window per item (system s, window
+// rows [r0, r0 + nrows)).  For each column block of the system in turn, the
+// block's vector entries (cbw rows of p, NR values each) are staged in LDS
+// and each wavefront streams whole slices of the window's block, one row per
+// lane, so the (value, 16-bit column) loads are contiguous across the
+// wavefront and a row needs no cross-lane reduction.  The rows are sorted
+// by segment length per block, so a lane's row changes from block to block:
+// the row's block sum is added to the window's row sums, kept in LDS beside
+// the vector block (the first block stores, later blocks add, in block
+// order).  After the last block the window's rows are complete: q = A p is
+// written once, with the p.q partial of the item.  No block partials go
+// through memory.  Slices go to wavefronts in a fixed snake order, so the
+// sums are reproducible.
+//
+// RESID = true computes the FP64 true residual instead: res = b - A x for the
+// window's rows and the partials of |res|^2.
 //
 // The entry stream is software-pipelined with sell_chunk: the loads of the
 // next chunk (or of the next slice's first chunk and its lane rows) are issued
-// before the current chunk's LDS gathers and FMAs, and the first chunk is
-// issued before the LDS fill, whose loads all go out before its LDS stores.
+// before the current chunk's LDS gathers and FMAs, and a block's first chunk
+// is issued before its LDS fill, whose loads all go out before its stores.
 //
 // One workgroup per CU walks its own list of items (block_item: item_blocks
-// + 1 list offsets, then the lists), with a barrier between items.  rev = 1
-// walks the list backwards: successive SpMMs alternate, so each starts on the
-// items the last one streamed last, which the memory-side cache still holds.
+// + 1 list offsets, then the lists), with a barrier after each block.
+// rev = 1 walks the list backwards: successive SpMMs alternate, so each starts
+// on the items the last one streamed last, which the memory-side cache still
+// holds.
 template <typename T, int NR, int BLOCK, int UNR, bool RESID>
 __global__ __launch_bounds__(BLOCK) void batch_spmm_sell_kernel(BatchDev bd,
                                                                 const T *__restrict__ sval,
                                                                 const T *__restrict__ p,
-                                                                T *__restrict__ rows_part,
-                                                                double *__restrict__ pq_part,
+                                                                const double *__restrict__ rhs,
+                                                                T *__restrict__ out,
+                                                                double *__restrict__ part,
                                                                 int rev) {
     extern __shared__ double2 lds_sell[];  // 16-byte aligned
     T *vlds = reinterpret_cast<T *>(lds_sell);
+    T *acc_lds = vlds + (size_t)bd.cbw * NR;  // [kSellWindow][NR] row sums of the window
     constexpr int kWaves = BLOCK / kWave;
-    const int *__restrict__ wl = bd.block_item + bd.item_blocks + 1;
-    const int k0 = bd.block_item[blockIdx.x], k1 = bd.block_item[blockIdx.x + 1];
-    for (int li = k0; li < k1; li++) {
-    const int it = wl[rev ? k0 + k1 - 1 - li : li];
-    const int s = bd.item[4 * it];
-    if (!RESID && system_done<NR>(bd.done, s)) continue;  // the same for every thread
-    const int b = bd.item[4 * it + 1], sl0 = bd.item[4 * it + 2];
-    const int nsl = bd.item[4 * it + 3] - sl0;
-    const int srow = bd.sys_row[s], ns = bd.sys_row[s + 1] - srow;
-    const int c0 = b * bd.cbw;
-    const int w = min(bd.cbw, ns - c0);
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform (scalar) index: every branch on the slice loop is scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int *__restrict__ off = bd.sell_off;
     const uint16_t *__restrict__ col = bd.sell_col;
     const int *__restrict__ lane_row = bd.sell_row;
-    // Slice of round rd for this wave (snake order: slices of a window are
-    // sorted widest first, and alternating the direction of each round evens
-    // out the width per wavefront), or -1.  Rounds are increasing in k, so
-    // after the first -1 there is none.
-    auto slice_of = [&](int rd) {
-        const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
-        return k < nsl ? sl0 + k : -1;
-    };
-    T na[UNR];
-    int nc[UNR];
-    int cur = slice_of(0), cbase = 0, cwidth = 0, nrow = -1;
-    if (cur >= 0) {
-        cbase = off[cur];
-        cwidth = (off[cur + 1] - cbase) / kWave;
-        nrow = lane_row[(size_t)cur * kWave + lane];
-        if (cwidth > 0) sell_chunk<T, UNR>(sval, col, cbase, cwidth, 0, lane, na, nc);
-    }
-    if constexpr (NR * sizeof(T) % 16 == 0) {  // rows of whole 16-byte units, 16-byte aligned
-        constexpr int kUnits = NR * (int)sizeof(T) / 16;
-        stage_block<BLOCK, (128 * 1024 / 16 + BLOCK - 1) / BLOCK>(
-            reinterpret_cast<const double2 *>(p + (size_t)(srow + c0) * NR), lds_sell, w * kUnits);
-    } else {
-        stage_block<BLOCK, (128 * 1024 / (int)sizeof(T) + BLOCK - 1) / BLOCK>(
-            p + (size_t)(srow + c0) * NR, vlds, w * NR);
-    }
-    __syncthreads();
-    T *part = rows_part + bd.sys_part[s] + (size_t)b * ns * NR;
-    double dot[NR];
-#pragma unroll
-    for (int q = 0; q < NR; q++) dot[q] = 0.0;
-    for (int rd = 0; cur >= 0; rd++) {
-        const int width = cwidth, base = cbase, row = nrow;
-        const int nxt = slice_of(rd + 1);
-        int xbase = 0, xwidth = 0;
-        if (nxt >= 0) {
-            xbase = off[nxt];
-            xwidth = (off[nxt + 1] - xbase) / kWave;
-        }
-        // p of the lane's row, for p.q (a padding lane reads row srow, unused)
-        VecT<T, NR> pv;
-        if constexpr (!RESID) pv = load_vec<T, NR>(p + (size_t)(row >= 0 ? row : srow) * NR);
-        T acc[NR];
-#pragma unroll
-        for (int q = 0; q < NR; q++) acc[q] = T(0);
-        const int nch = (width + UNR - 1) / UNR;
-        for (int ch = 0; ch < nch; ch++) {
-            T a[UNR];
-            int c[UNR];
-#pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                a[u] = na[u];
-                c[u] = nc[u];
+    const int *__restrict__ wl = bd.block_item + bd.item_blocks + 1;
+    const int k0 = bd.block_item[blockIdx.x], k1 = bd.block_item[blockIdx.x + 1];
+    for (int li = k0; li < k1; li++) {
+        const int it = wl[rev ? k0 + k1 - 1 - li : li];
+        const int s = bd.item[4 * it];
+        if (!RESID && system_done<NR>(bd.done, s)) continue;  // the same for every thread
+        const int r0 = bd.item[4 * it + 1], nrows = bd.item[4 * it + 2];
+        const int *__restrict__ ranges = bd.item_sl + bd.item[4 * it + 3];
+        const int srow = bd.sys_row[s], ns = bd.sys_row[s + 1] - srow, nb = bd.sys_nb[s];
+        const int wrow = srow + r0;  // first global row of the window
+        for (int b = 0; b < nb; b++) {
+            const int sl0 = ranges[2 * b], nsl = ranges[2 * b + 1] - sl0;
+            const int c0 = b * bd.cbw;
+            const int w = min(bd.cbw, ns - c0);
+            // Slice of round rd for this wave (snake order: slices of a window
+            // are sorted widest first, and alternating the direction of each
+            // round evens out the width per wavefront), or -1.  Rounds are
+            // increasing in k, so after the first -1 there is none.
+            auto slice_of = [&](int rd) {
+                const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
+                return k < nsl ? sl0 + k : -1;
+            };
+            T na[UNR];
+            int nc[UNR];
+            int cur = slice_of(0), cbase = 0, cwidth = 0, nrow = -1;
+            if (cur >= 0) {
+                cbase = off[cur];
+                cwidth = (off[cur + 1] - cbase) / kWave;
+                nrow = lane_row[(size_t)cur * kWave + lane];
+                if (cwidth > 0) sell_chunk<T, UNR>(sval, col, cbase, cwidth, 0, lane, na, nc);
             }
-            sell_trim<T, UNR>(width - ch * UNR, a, c);
-            if (ch + 1 < nch) {
-                sell_chunk<T, UNR>(sval, col, base, width, (ch + 1) * UNR, lane, na, nc);
-            } else if (nxt >= 0) {
-                nrow = lane_row[(size_t)nxt * kWave + lane];
-                if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
+            if constexpr (NR * sizeof(T) % 16 == 0) {  // rows of whole 16-byte units, aligned
+                constexpr int kUnits = NR * (int)sizeof(T) / 16;
+                stage_block<BLOCK, (128 * 1024 / 16 + BLOCK - 1) / BLOCK>(
+                    reinterpret_cast<const double2 *>(p + (size_t)(srow + c0) * NR), lds_sell,
+                    w * kUnits);
+            } else {
+                stage_block<BLOCK, (128 * 1024 / (int)sizeof(T) + BLOCK - 1) / BLOCK>(
+                    p + (size_t)(srow + c0) * NR, vlds, w * NR);
             }
+            __syncthreads();
+            for (int rd = 0; cur >= 0; rd++) {
+                const int width = cwidth, base = cbase, row = nrow;
+                const int nxt = slice_of(rd + 1);
+                int xbase = 0, xwidth = 0;
+                if (nxt >= 0) {
+                    xbase = off[nxt];
+                    xwidth = (off[nxt + 1] - xbase) / kWave;
+                }
+                T acc[NR];
 #pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const VecT<T, NR> xv = load_vec<T, NR>(vlds + c[u] * NR);
+                for (int q = 0; q < NR; q++) acc[q] = T(0);
+                const int nch = (width + UNR - 1) / UNR;
+                for (int ch = 0; ch < nch; ch++) {
+                    T a[UNR];
+                    int c[UNR];
 #pragma unroll
-                for (int q = 0; q < NR; q++) acc[q] = fma(a[u], xv.v[q], acc[q]);
+                    for (int u = 0; u < UNR; u++) {
+                        a[u] = na[u];
+                        c[u] = nc[u];
+                    }
+                    sell_trim<T, UNR>(width - ch * UNR, a, c);
+                    if (ch + 1 < nch) {
+                        sell_chunk<T, UNR>(sval, col, base, width, (ch + 1) * UNR, lane, na, nc);
+                    } else if (nxt >= 0) {
+                        nrow = lane_row[(size_t)nxt * kWave + lane];
+                        if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; u++) {
+                        const VecT<T, NR> xv = load_vec<T, NR>(vlds + c[u] * NR);
+#pragma unroll
+                        for (int q = 0; q < NR; q++) acc[q] = fma(a[u], xv.v[q], acc[q]);
+                    }
+                }
+                if (nch == 0 && nxt >= 0) {
+                    nrow = lane_row[(size_t)nxt * kWave + lane];
+                    if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
+                }
+                if (row >= 0) {  // every row of the window is in one slice of each block
+                    T *dst = acc_lds + (size_t)(row - wrow) * NR;
+                    VecT<T, NR> sum;
+                    if (b == 0) {
+#pragma unroll
+                        for (int q = 0; q < NR; q++) sum.v[q] = acc[q];
+                    } else {
+                        sum = load_vec<T, NR>(dst);
+#pragma unroll
+                        for (int q = 0; q < NR; q++) sum.v[q] += acc[q];
+                    }
+                    store_vec<T, NR>(dst, sum);
+                }
+                cur = nxt;
+                cbase = xbase;
+                cwidth = xwidth;
+            }
+            __syncthreads();  // the vector block is free again; the row sums are visible
+        }
+        double dot[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) dot[q] = 0.0;
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+            const VecT<T, NR> sum = load_vec<T, NR>(acc_lds + (size_t)i * NR);
+            const size_t at = (size_t)(wrow + i) * NR;
+            if constexpr (RESID) {
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    const double rv = rhs[at + q] - (double)sum.v[q];
+                    out[at + q] = (T)rv;
+                    dot[q] += rv * rv;
+                }
+            } else {
+                const VecT<T, NR> pv = load_vec<T, NR>(p + at);
+                store_vec<T, NR>(out + at, sum);
+#pragma unroll
+                for (int q = 0; q < NR; q++) dot[q] += (double)pv.v[q] * (double)sum.v[q];
             }
         }
-        if (nch == 0 && nxt >= 0) {
-            nrow = lane_row[(size_t)nxt * kWave + lane];
-            if (xwidth > 0) sell_chunk<T, UNR>(sval, col, xbase, xwidth, 0, lane, na, nc);
-        }
-        if (row >= 0) {
-            VecT<T, NR> out;
-#pragma unroll
-            for (int q = 0; q < NR; q++) {
-                out.v[q] = acc[q];
-                if constexpr (!RESID) dot[q] += (double)pv.v[q] * (double)acc[q];
-            }
-            store_vec<T, NR>(part + (size_t)(row - srow) * NR, out);
-        }
-        cur = nxt;
-        cbase = xbase;
-        cwidth = xwidth;
+        block_sum_store<NR, BLOCK>(dot, part + (size_t)it * NR);
+        __syncthreads();  // the row sums and the reduction slots are free again
     }
-    if constexpr (!RESID) block_sum_store<NR, BLOCK>(dot, pq_part + (size_t)it * NR);
-    __syncthreads();  // the vector block and the reduction slots are free again
-    }
-}
-
-// FP64 true residual from the SELL row partials of A x: res = b - sum over
-// the system's column blocks (in order) of the partials; |res|^2 partials per
-// tile.
-template <int NR>
-__global__ __launch_bounds__(kBlock) void batch_resid_combine_kernel(BatchDev bd,
-                                                                     const double *__restrict__ rows_part,
-                                                                     const double *__restrict__ b,
-                                                                     double *__restrict__ res,
-                                                                     double *__restrict__ part) {
-    const int t = blockIdx.x;
-    const int s = bd.tile_sys[t];
-    const int row0 = bd.tile_row[t], row1 = bd.tile_row[t + 1];
-    const int srow = bd.sys_row[s], ns = bd.sys_row[s + 1] - srow, nb = bd.sys_nb[s];
-    const double *sp = rows_part + bd.sys_part[s];
-    double nrm[NR];
-#pragma unroll
-    for (int k = 0; k < NR; k++) nrm[k] = 0.0;
-    for (int row = row0 + threadIdx.x; row < row1; row += kBlock) {
-        const size_t local = (size_t)(row - srow) * NR;
-        double ax[NR];
-#pragma unroll
-        for (int k = 0; k < NR; k++) ax[k] = sp[local + k];
-#pragma unroll 8
-        for (int blk = 1; blk < nb; blk++)
-#pragma unroll
-            for (int k = 0; k < NR; k++) ax[k] += sp[(size_t)blk * ns * NR + local + k];
-#pragma unroll
-        for (int k = 0; k < NR; k++) {
-            const size_t at = (size_t)row * NR + k;
-            const double rv = b[at] - ax[k];
-            res[at] = rv;
-            nrm[k] += rv * rv;
-        }
-    }
-    block_sum_store<NR>(nrm, part + (size_t)t * NR);
 }
 
 // Residual of x = 0: res = b; |b|^2 partials per tile.
@@ -513,13 +511,16 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(BatchDev bd, const
     double acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) acc[k] = 0.0;
-    const int srow = bd.cb ? bd.sys_row[s] : 0;
-    const int ns = bd.cb ? bd.sys_row[s + 1] - srow : 0;
-    const int nb = bd.cb ? bd.sys_nb[s] : 0;
-    const T *part = bd.cb ? rows_part + bd.sys_part[s] : nullptr;
+    // column blocks in CSR order: q = sum of the blocks' row partials; the
+    // SELL SpMM writes q itself
+    const bool combine = bd.cb && !bd.sell;
+    const int srow = combine ? bd.sys_row[s] : 0;
+    const int ns = combine ? bd.sys_row[s + 1] - srow : 0;
+    const int nb = combine ? bd.sys_nb[s] : 0;
+    const T *part = combine ? rows_part + bd.sys_part[s] : nullptr;
     for (int row = row0 + threadIdx.x; row < row1; row += kBlock) {
         T qv[NR];
-        if (bd.cb) {  // q = sum of the column blocks' row partials, blocks in order
+        if (combine) {  // blocks in order
             const size_t local = (size_t)(row - srow) * NR;
 #pragma unroll
             for (int k = 0; k < NR; k++) qv[k] = part[local + k];
@@ -729,8 +730,8 @@ template <typename T, int NR, int LPR>
 void launch_spmm(const BatchDev &bd, const BatchVecs<T> &v, int rev, hipStream_t st) {
     if (bd.sell) {
         batch_spmm_sell_kernel<T, NR, kSellBlock, sell_unr<T, NR>(), false>
-            <<<bd.item_blocks, kSellBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
-                bd, v.sval, v.p, v.rows_part, v.part_pq, rev);
+            <<<bd.item_blocks, kSellBlock, sell_lds_bytes(bd.cbw, NR, sizeof(T)), st>>>(
+                bd, v.sval, v.p, nullptr, v.q, v.part_pq, rev);
     } else if (bd.cb) {
         batch_spmm_cb_kernel<T, NR, LPR><<<bd.items, kBlock, (size_t)bd.cbw * NR * sizeof(T), st>>>(
             bd, v.val, v.p, v.rows_part, v.part_pq);
@@ -854,22 +855,21 @@ hipError_t batch_residual(const BatchDev &bd, const double *val, const double *b
 }
 
 hipError_t batch_residual_sell(const BatchDev &bd, const double *sval, const double *b,
-                               const double *x, double *rows_part, double *res, double *part,
-                               double *sys_out, hipStream_t st) {
-    hipError_t e = with_nr(bd.nr, [&](auto nrc) {
+                               const double *x, double *res, double *part, double *sys_out,
+                               hipStream_t st) {
+    // |res|^2 partials are per item: the system sums run over sys_item
+    BatchDev by_item = bd;
+    by_item.sys_tile = bd.sys_item;
+    return with_nr(bd.nr, [&](auto nrc) {
         constexpr int NR = decltype(nrc)::value;
         batch_spmm_sell_kernel<double, NR, kSellBlock, sell_unr<double, NR>(), true>
-            <<<bd.item_blocks, kSellBlock, (size_t)bd.cbw * NR * sizeof(double), st>>>(
-                bd, sval, x, rows_part, nullptr, 0);
+            <<<bd.item_blocks, kSellBlock, sell_lds_bytes(bd.cbw, NR, sizeof(double)), st>>>(
+                bd, sval, x, b, res, part, 0);
         hipError_t e2 = hipGetLastError();
         if (e2 != hipSuccess) return e2;
-        batch_resid_combine_kernel<NR><<<bd.tiles, kBlock, 0, st>>>(bd, rows_part, b, res, part);
-        e2 = hipGetLastError();
-        if (e2 != hipSuccess) return e2;
-        batch_system_sum_kernel<NR><<<bd.nsys, kBlock, 0, st>>>(bd, part, sys_out);
+        batch_system_sum_kernel<NR><<<bd.nsys, kBlock, 0, st>>>(by_item, part, sys_out);
         return hipGetLastError();
     });
-    return e;
 }
 
 hipError_t batch_residual_zero(const BatchDev &bd, const double *b, double *res, double *part,

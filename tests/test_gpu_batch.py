@@ -64,8 +64,8 @@ def test_batch_launch_shapes_agree(gpu):
             bt.set_rhs(rhs)
             st, _, _ = bt.solve(RTOL, 3000)
             assert st.layout == layout and st.converged == st.solves
-            if layout == 3:  # whole 2048-row sorting windows per workgroup
-                assert st.rows_per_tile == max(2048, rt // 2048 * 2048)
+            if layout == 3:  # one 2048-row sorting window per item, whatever is asked
+                assert st.rows_per_tile == 2048
             sols.append(np.concatenate(bt.solution()))
     for x in sols[1:]:
         assert np.max(np.abs(x - sols[0])) <= 1e-9 * np.max(np.abs(sols[0]))
@@ -118,9 +118,9 @@ def test_batch_many_systems_xcd_placement(gpu):
 
 
 def test_batch_sell_many_blocks_and_windows(gpu):
-    # systems wide enough for several column blocks (8192 columns with four
-    # FP32 right-hand sides, 4096 with four FP64 ones) and several 2048-row
-    # sorting windows per block, checked against the oracle's CG
+    # systems wide enough for several column blocks (8064 columns with four
+    # FP32 right-hand sides, 3008 with four FP64 ones) and several 2048-row
+    # sorting windows, checked against the oracle's CG
     params = [(9000, 9, 20.0, 7.0), (20000, 10, 20.0, 8.0), (33000, 8, 20.0, 9.0)]
     mats = gpu.make_many(params, workers=3)
     rhs = _rhs(mats, 4)
