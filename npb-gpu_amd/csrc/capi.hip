@@ -131,7 +131,7 @@ struct npbcg_ctx {
         float *gh_zero = nullptr, *gh_learned = nullptr;  // (g, h), 2n each
         double *pz = nullptr;    // z = M^-1 r work vectors
         float *pz32 = nullptr;
-        std::vector<int> h_off;
+        size_t slots = 0;             // entries of the operator
         npbcg_dev::PrecondOp op;
         int loaded = -1;         // preconditioner kind whose values op holds
     } g;
@@ -1406,7 +1406,8 @@ int ensure_graph_device(npbcg_ctx *c) {
     return NPBCG_OK;
 }
 
-// The CSR operator of M^-1 (structure built on the host from the graph).
+// The CSR operator of M^-1 (structure built on the device from the graph;
+// the host reads only the slot count).
 int ensure_row_graph(npbcg_ctx *c) {
     auto &g = c->g;
     if (g.built) return NPBCG_OK;
@@ -1414,44 +1415,15 @@ int ensure_row_graph(npbcg_ctx *c) {
     if (rc != NPBCG_OK) return rc;
     const int n = c->n;
     const int K = npbcg_dev::kNbr;
-    const size_t nk = (size_t)n * K;
-    std::vector<int> nbr(nk), tptr(n + 1);
-    std::vector<float> nval(nk);
-    NPBCG_HIP(hipMemcpyAsync(nbr.data(), g.nbr, sizeof(int) * nk, hipMemcpyDeviceToHost,
-                             c->stream));
-    NPBCG_HIP(hipMemcpyAsync(nval.data(), g.nval, sizeof(float) * nk, hipMemcpyDeviceToHost,
-                             c->stream));
-    NPBCG_HIP(hipMemcpyAsync(tptr.data(), g.tptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost,
-                             c->stream));
+    int edges = 0;
+    NPBCG_HIP(hipMemcpyAsync(&edges, g.tptr + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     NPBCG_HIP(hipStreamSynchronize(c->stream));
-    const int edges = tptr[n];
-    std::vector<int> tsrc(std::max(edges, 1));
-    std::vector<float> tval(std::max(edges, 1));
-    if (edges > 0) {
-        NPBCG_HIP(hipMemcpyAsync(tsrc.data(), g.tsrc, sizeof(int) * edges, hipMemcpyDeviceToHost,
-                                 c->stream));
-        NPBCG_HIP(hipMemcpyAsync(tval.data(), g.tval, sizeof(float) * edges,
-                                 hipMemcpyDeviceToHost, c->stream));
-        NPBCG_HIP(hipStreamSynchronize(c->stream));
-    }
-    // row j: [diagonal | K out-neighbours | in-neighbours]
-    g.h_off.assign(n + 1, 0);
-    for (int j = 0; j < n; j++) g.h_off[j + 1] = g.h_off[j] + 1 + K + tptr[j + 1] - tptr[j];
-    const size_t slots = (size_t)g.h_off[n];
-    std::vector<int> col(slots, 0);
-    std::vector<float> src(slots, 0.0f);
-    for (int j = 0; j < n; j++) {
-        size_t at = (size_t)g.h_off[j];
-        col[at++] = j;
-        for (int t = 0; t < K; t++, at++) {
-            col[at] = nbr[(size_t)j * K + t];
-            src[at] = nval[(size_t)j * K + t];
-        }
-        for (int e = tptr[j]; e < tptr[j + 1]; e++, at++) {
-            col[at] = tsrc[e];
-            src[at] = tval[e];
-        }
-    }
+    // row j: [diagonal | K out-neighbours | in-neighbours], first slot j (1 + K) + tptr[j]
+    const long long slots_ll = (long long)n * (1 + K) + edges;
+    if (slots_ll > (long long)INT32_MAX)
+        return set_error(NPBCG_ERR_CAPACITY, "preconditioner operator exceeds 32-bit offsets");
+    const size_t slots = (size_t)slots_ll;
+    g.slots = slots;
     NPBCG_HIP(hipMalloc(&g.op_off, sizeof(int) * (n + 1)));
     NPBCG_HIP(hipMalloc(&g.op_col, sizeof(int) * slots));
     NPBCG_HIP(hipMalloc(&g.op_src, sizeof(float) * slots));
@@ -1460,16 +1432,11 @@ int ensure_row_graph(npbcg_ctx *c) {
     NPBCG_HIP(hipMalloc(&g.gh_learned, sizeof(float) * 2 * n));
     NPBCG_HIP(hipMalloc(&g.pz, sizeof(double) * c->vstride));
     NPBCG_HIP(hipMalloc(&g.pz32, sizeof(float) * c->vstride));
-    NPBCG_HIP(hipMemcpyAsync(g.op_off, g.h_off.data(), sizeof(int) * (n + 1),
-                             hipMemcpyHostToDevice, c->stream));
-    NPBCG_HIP(hipMemcpyAsync(g.op_col, col.data(), sizeof(int) * slots, hipMemcpyHostToDevice,
-                             c->stream));
-    NPBCG_HIP(hipMemcpyAsync(g.op_src, src.data(), sizeof(float) * slots, hipMemcpyHostToDevice,
-                             c->stream));
+    NPBCG_HIP(npbcg_dev::launch_build_op(g.nbr, g.nval, g.tptr, g.tsrc, g.tval, n, g.op_off,
+                                         g.op_col, g.op_src, c->stream));
     NPBCG_HIP(hipMemsetAsync(g.gh_learned, 0, sizeof(float) * 2 * n, c->stream));
     NPBCG_HIP(hipMemsetAsync(g.pz, 0, sizeof(double) * c->vstride, c->stream));
     NPBCG_HIP(hipMemsetAsync(g.pz32, 0, sizeof(float) * c->vstride, c->stream));
-    NPBCG_HIP(hipStreamSynchronize(c->stream));  // host vectors go out of scope
     g.op.n = n;
     g.op.off = g.op_off;
     g.op.col = g.op_col;
@@ -1609,12 +1576,15 @@ int npbcg_get_preconditioner(npbcg_ctx *c, double *minv) {
         std::memcpy(minv, c->h_stage, bytes);
     } else if (c->pbs.pop) {  // entry 0 of every row of the operator
         const auto &g = c->g;
-        std::vector<double> val(g.h_off.back());
+        std::vector<double> val(g.slots);
+        std::vector<int> off(c->n + 1);
         NPBCG_HIP(hipMemcpyAsync(val.data(), g.op_val, sizeof(double) * val.size(),
+                                 hipMemcpyDeviceToHost, c->stream));
+        NPBCG_HIP(hipMemcpyAsync(off.data(), g.op_off, sizeof(int) * off.size(),
                                  hipMemcpyDeviceToHost, c->stream));
         NPBCG_HIP(hipStreamSynchronize(c->stream));
         for (int j = 0; j < c->n; j++)
-            minv[j] = val[(size_t)g.h_off[j]];
+            minv[j] = val[(size_t)off[j]];
     } else {
         for (int j = 0; j < c->n; j++) minv[j] = 1.0;
     }

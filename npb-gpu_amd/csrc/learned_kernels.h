@@ -88,6 +88,13 @@ hipError_t launch_graph_features(const int *rowstr, const int *colidx, const dou
 hipError_t launch_transpose(const int *nbr, const float *wgt, const float *nval, int n, int *cnt,
                             int *tptr, int *tsrc, float *tw, float *tval, hipStream_t st);
 
+// CSR structure of the operator from the graph: off[n + 1], and per row
+// [diagonal | K out-neighbours | in-neighbours] in col, neighbour values in
+// src (0 for the diagonal).  off[n] = n (1 + K) + tptr[n].
+hipError_t launch_build_op(const int *nbr, const float *nval, const int *tptr, const int *tsrc,
+                           const float *tval, int n, int *off, int *col, float *src,
+                           hipStream_t st);
+
 // Feature mean/std over the rows of one system: stats[2*kFeat].
 hipError_t launch_feature_stats(const float *feat, int n, float *stats, hipStream_t st);
 

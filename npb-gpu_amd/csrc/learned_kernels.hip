@@ -860,6 +860,47 @@ hipError_t launch_transpose(const int *nbr, const float *wgt, const float *nval,
     return hipGetLastError();
 }
 
+// Structure of the operator M^-1 = D (I + G N + N^T H) D as a CSR: row j is
+// [j | its K out-neighbours | its in-neighbours (transpose order)], so its
+// first entry sits at j (1 + K) + tptr[j] and no scan is needed.  src holds
+// the neighbour values (0 for the diagonal slot).  One thread per row, grid
+// stride (row n only writes off[n]).
+__global__ __launch_bounds__(kBlock) void build_op_kernel(const int *__restrict__ nbr,
+                                                          const float *__restrict__ nval,
+                                                          const int *__restrict__ tptr,
+                                                          const int *__restrict__ tsrc,
+                                                          const float *__restrict__ tval, int n,
+                                                          int *__restrict__ off,
+                                                          int *__restrict__ col,
+                                                          float *__restrict__ src) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j <= n; j += gridDim.x * kBlock) {
+        const int t0 = tptr[j];
+        const int base = j * (1 + kNbr) + t0;
+        off[j] = base;
+        if (j == n) break;
+        col[base] = j;
+        src[base] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < kNbr; t++) {
+            col[base + 1 + t] = nbr[(size_t)j * kNbr + t];
+            src[base + 1 + t] = nval[(size_t)j * kNbr + t];
+        }
+        const int t1 = tptr[j + 1];
+        for (int e = t0; e < t1; e++) {
+            col[base + 1 + kNbr + e - t0] = tsrc[e];
+            src[base + 1 + kNbr + e - t0] = tval[e];
+        }
+    }
+}
+
+hipError_t launch_build_op(const int *nbr, const float *nval, const int *tptr, const int *tsrc,
+                           const float *tval, int n, int *off, int *col, float *src,
+                           hipStream_t st) {
+    build_op_kernel<<<blocks_for((long long)n + 1), kBlock, 0, st>>>(nbr, nval, tptr, tsrc, tval,
+                                                                    n, off, col, src);
+    return hipGetLastError();
+}
+
 hipError_t launch_feature_stats(const float *feat, int n, float *stats, hipStream_t st) {
     feature_stats_kernel<<<1, 1024, 0, st>>>(feat, n, stats);
     return hipGetLastError();
