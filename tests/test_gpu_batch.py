@@ -138,6 +138,36 @@ def test_batch_sell_many_blocks_and_windows(gpu):
                 assert np.max(np.abs(xs[s][:, k] - ox)) <= 1e-8 * np.max(np.abs(ox))
 
 
+@pytest.mark.parametrize("mode,nrhs", [(1, 4), (0, 4), (0, 1), (1, 2)])
+def test_batch_sell_window_edges(gpu, mode, nrhs):
+    # SELL window items at the edges of their shapes: systems smaller than one
+    # 64-row slice, exactly one 2048-row sorting window and one row more, one
+    # column block (3008 columns for four FP64 right-hand sides) and one
+    # column more, several blocks with a one-column last block, and more than
+    # one FP32 block (8064 columns).  Every solution is checked against the
+    # oracle's CG and the host residual.
+    params = [(40, 5, 20.0, 11.0), (2048, 7, 20.0, 12.0), (2049, 8, 20.0, 13.0),
+              (3008, 7, 20.0, 14.0), (3009, 9, 20.0, 15.0), (6017, 8, 20.0, 16.0),
+              (8065, 7, 20.0, 17.0), (100, 6, 20.0, 18.0)]
+    mats = gpu.make_many(params, workers=4)
+    rhs = _rhs(mats, nrhs)
+    with gpu.Batch(mats, nrhs=nrhs, mode=mode, layout=3) as bt:
+        bt.set_rhs(rhs)
+        st, steps, rel = bt.solve(RTOL, 3000)
+        xs = bt.solution()
+    print(f"mode {mode} nrhs {nrhs}: {st.as_dict()}; steps {steps.ravel().tolist()}")
+    assert st.layout == 3 and st.converged == len(mats) * nrhs and st.max_relres <= RTOL
+    for s, m in enumerate(mats):
+        om = _ocsr(m)
+        for k in range(nrhs):
+            b = rhs[s][:, k]
+            x = xs[s][:, k]
+            host = np.linalg.norm(b - oracle.spmv(om, x)) / np.linalg.norm(b)
+            assert host <= RTOL and abs(host - rel[s, k]) <= 1e-3 * RTOL + 0.05 * host
+            ox, _, _ = oracle.cg_solve(om, b, rtol=RTOL, maxit=5000)
+            assert np.max(np.abs(x - ox)) <= 1e-8 * np.max(np.abs(ox))
+
+
 @pytest.mark.parametrize("mode", [1, 0])
 def test_batch_full_workload(gpu, mode):
     # The full batched workload: 64 systems, n ~ U{2000..40000}, nonzer ~
