@@ -358,41 +358,68 @@ __global__ __launch_bounds__(kBlock) void build_x0_kernel(const float *__restric
     }
 }
 
-// Y[n x 64] = ReLU(X[n x K] W + b): one wavefront per 16 rows computes four
-// 16x16 output blocks, K in steps of 32.  Wt is W transposed (64 x K, BF16).
-// Lane l supplies A[l%16][8(l/16)..+7] and B[8(l/16)..+7][l%16] and receives
+// Y[n x 64] = ReLU(X[n x K] W + b): a wavefront computes the four 16x16
+// output blocks of kFwdTiles consecutive 16-row tiles, K in steps of 32.  Wt
+// is W transposed (64 x K, BF16); the wavefront loads its B fragments (and
+// the bias) once and keeps them in registers for all its tiles, so the
+// weights are read once per wavefront rather than once per tile (at K = 128
+// they are 16 KB against the tile's 4 KB of X).  The next tile's A fragments
+// are loaded before the current tile's products.  Lane l supplies
+// A[l%16][8(l/16)..+7] and B[8(l/16)..+7][l%16] and receives
 // D[4(l/16)..+3][l%16] of each 16x16x32 product.
+constexpr int kFwdTiles = 4;
 template <int K>
 __global__ __launch_bounds__(kBlock) void gemm_fwd_kernel(const __bf16 *__restrict__ x, int ldx,
                                                           int n, const __bf16 *__restrict__ wt,
                                                           const float *__restrict__ bias,
                                                           __bf16 *__restrict__ y, int ldy) {
+    constexpr int KS = K / 32;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int row0 = (blockIdx.x * kWavesPerBlock + wave) * 16;
-    if (row0 >= n) return;
-    const int ar = row0 + (lane & 15);
+    const int tile0 = (blockIdx.x * kWavesPerBlock + wave) * kFwdTiles;
+    if (tile0 * 16 >= n) return;
     const int kq = 8 * (lane >> 4);
-    f32x4 acc[4] = {};
-#pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 av = {};
-        if (ar < n) av = *reinterpret_cast<const bf16x8 *>(x + (size_t)ar * ldx + k0 + kq);
-#pragma unroll
-        for (int jb = 0; jb < 4; jb++) {
-            const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(
-                wt + (size_t)(jb * 16 + (lane & 15)) * K + k0 + kq);
-            acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[jb], 0, 0, 0);
-        }
-    }
+    bf16x8 bw[KS][4];
+    float bj[4];
 #pragma unroll
     for (int jb = 0; jb < 4; jb++) {
-        const int j = jb * 16 + (lane & 15);
-        const float bj = bias[j];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int row = row0 + 4 * (lane >> 4) + t;
-            if (row < n) y[(size_t)row * ldy + j] = (__bf16)fmaxf(acc[jb][t] + bj, 0.0f);
+        for (int k = 0; k < KS; k++)
+            bw[k][jb] = *reinterpret_cast<const bf16x8 *>(
+                wt + (size_t)(jb * 16 + (lane & 15)) * K + 32 * k + kq);
+        bj[jb] = bias[jb * 16 + (lane & 15)];
+    }
+    auto load_a = [&](int row0, bf16x8 (&av)[KS]) {
+        const int ar = row0 + (lane & 15);
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            av[k] = bf16x8{};
+            if (ar < n) av[k] = *reinterpret_cast<const bf16x8 *>(x + (size_t)ar * ldx + 32 * k + kq);
+        }
+    };
+    bf16x8 av[KS];
+    load_a(tile0 * 16, av);
+    for (int t = 0; t < kFwdTiles; t++) {
+        const int row0 = (tile0 + t) * 16;
+        if (row0 >= n) break;  // the same for the whole wavefront
+        bf16x8 cur[KS];
+#pragma unroll
+        for (int k = 0; k < KS; k++) cur[k] = av[k];
+        if (t + 1 < kFwdTiles) load_a(row0 + 16, av);
+        f32x4 acc[4] = {};
+#pragma unroll
+        for (int k = 0; k < KS; k++)
+#pragma unroll
+            for (int jb = 0; jb < 4; jb++)
+                acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[k], bw[k][jb], acc[jb], 0, 0, 0);
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) {
+            const int j = jb * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int row = row0 + 4 * (lane >> 4) + q;
+                if (row < n) y[(size_t)row * ldy + j] = (__bf16)fmaxf(acc[jb][q] + bj[jb], 0.0f);
+            }
         }
     }
 }
@@ -936,7 +963,7 @@ hipError_t launch_net_forward(const NetBuffers &nb, const NetSystem &sys, const 
     build_x0_kernel<<<blocks_for((long long)n * kK0), kBlock, 0, st>>>(sys.feat, nb.norm, n,
                                                                       nb.x0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int gb = exact_blocks(n, 16 * kWavesPerBlock);
+    const int gb = exact_blocks(n, 16 * kFwdTiles * kWavesPerBlock);
     gemm_fwd_kernel<kK0><<<gb, kBlock, 0, st>>>(nb.x0, kK0, n, wb + kWtTransposed,
                                                 params + kOffB0, nb.x[0], kXW);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -974,7 +1001,7 @@ hipError_t launch_net_backward(const NetBuffers &nb, const NetSystem &sys, const
     if ((e = hipGetLastError()) != hipSuccess) return e;
     reduce_parts_kernel<<<1, kBlock, 0, st>>>(nb.rpart, rb, kReadout, kReadout, grad + kOffR);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int gb = exact_blocks(n, 16 * kWavesPerBlock);
+    const int gb = exact_blocks(n, 16 * kWavesPerBlock);  // one 16-row tile per wavefront
     const int wgb = std::min(nb.wparts, blocks_for(n, 32));
     for (int l = kLayers - 1; l >= 0; l--) {
         gemm_bwd_weight_kernel<kXW><<<wgb, kBlock, 0, st>>>(nb.x[l], kXW, n, nb.dz, nb.wpart);
