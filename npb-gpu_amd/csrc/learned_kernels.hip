@@ -740,15 +740,35 @@ __global__ __launch_bounds__(kBlock) void gemm_bwd_weight_kernel(const __bf16 *_
     if (threadIdx.x < kHid) out[K * kHid + threadIdx.x] = db;
 }
 
-// out[i] = sum over p < parts of part[p * stride + i], in order of p.
-__global__ __launch_bounds__(kBlock) void reduce_parts_kernel(const float *__restrict__ part,
-                                                              int parts, int stride, int count,
-                                                              float *__restrict__ out) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
-        float s = 0.f;
-        for (int p = 0; p < parts; p++) s += part[(size_t)p * stride + i];
-        out[i] = s;
+// out[i] = sum over p of part[p * stride + i], i < count, p in increasing
+// order within each of kRedWaves interleaved groups, the groups then added in
+// order (a fixed order: the same bits every run).  A 1024-thread workgroup
+// takes 64 consecutive columns; wavefront w sums the parts p = w, w + 16, ...
+// of its lane's column, so the loads are coalesced and 16 wavefronts share a
+// column's parts.
+constexpr int kRedWaves = 16;
+__global__ __launch_bounds__(kRedWaves * kWave) void reduce_parts_kernel(
+    const float *__restrict__ part, int parts, int stride, int count, float *__restrict__ out) {
+    __shared__ float ws[kRedWaves][kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int i = blockIdx.x * kWave + lane;
+    float s = 0.f;
+    if (i < count)
+        for (int p = wave; p < parts; p += kRedWaves) s += part[(size_t)p * stride + i];
+    ws[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && i < count) {
+        float t = ws[0][lane];
+        for (int w = 1; w < kRedWaves; w++) t += ws[w][lane];
+        out[i] = t;
     }
+}
+
+hipError_t launch_reduce_parts(const float *part, int parts, int stride, int count, float *out,
+                               hipStream_t st) {
+    reduce_parts_kernel<<<(count + kWave - 1) / kWave, kRedWaves * kWave, 0, st>>>(part, parts,
+                                                                                stride, count, out);
+    return hipGetLastError();
 }
 
 // Adam step on the FP32 master parameters (bias-corrected moments).
@@ -999,16 +1019,16 @@ hipError_t launch_net_backward(const NetBuffers &nb, const NetSystem &sys, const
     readout_bwd_kernel<<<rb, kBlock, 0, st>>>(nb.h3, params + kOffR, gmax, n, nb.dgh, nb.dz,
                                               nb.rpart);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    reduce_parts_kernel<<<1, kBlock, 0, st>>>(nb.rpart, rb, kReadout, kReadout, grad + kOffR);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_reduce_parts(nb.rpart, rb, kReadout, kReadout, grad + kOffR, st)) != hipSuccess)
+        return e;
     const int gb = exact_blocks(n, 16 * kWavesPerBlock);  // one 16-row tile per wavefront
     const int wgb = std::min(nb.wparts, blocks_for(n, 32));
     for (int l = kLayers - 1; l >= 0; l--) {
         gemm_bwd_weight_kernel<kXW><<<wgb, kBlock, 0, st>>>(nb.x[l], kXW, n, nb.dz, nb.wpart);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        reduce_parts_kernel<<<blocks_for(kLayerStride), kBlock, 0, st>>>(
-            nb.wpart, wgb, kLayerStride, kLayerStride, grad + kOffW1 + l * kLayerStride);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = launch_reduce_parts(nb.wpart, wgb, kLayerStride, kLayerStride,
+                                     grad + kOffW1 + l * kLayerStride, st)) != hipSuccess)
+            return e;
         gemm_bwd_data_kernel<<<gb, kBlock, 0, st>>>(
             nb.dz, n, wb + kWtRowMajor + kK0 * kHid + l * kXW * kHid, nb.dx);
         if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1018,9 +1038,8 @@ hipError_t launch_net_backward(const NetBuffers &nb, const NetSystem &sys, const
     }
     gemm_bwd_weight_kernel<kK0><<<wgb, kBlock, 0, st>>>(nb.x0, kK0, n, nb.dz, nb.wpart);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    reduce_parts_kernel<<<blocks_for(kK0 * kHid + kHid), kBlock, 0, st>>>(
-        nb.wpart, wgb, kK0 * kHid + kHid, kK0 * kHid + kHid, grad + kOffW0);
-    return hipGetLastError();
+    return launch_reduce_parts(nb.wpart, wgb, kK0 * kHid + kHid, kK0 * kHid + kHid, grad + kOffW0,
+                               st);
 }
 
 hipError_t launch_adam(float *p, const float *g, float *m1, float *m2, int count, float lr,

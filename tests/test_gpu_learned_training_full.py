@@ -7,10 +7,10 @@ from plain CG, cg.cu:689-772).
 
 Every system's pair loss is taken before its own Adam steps, so on data the
 network has not seen, relative to the loss of the network's base g = h = 0 (the
-Neumann form).  The shipped run's record: mean ratio 1.64 over the first 200
-systems (the untrained readout), medians 0.836, 0.817, 0.805 and 0.798 over the
-four quarters, median 0.794 over the last 200 (a few systems reach ratios of 2
-to 5 at any stage, so medians carry the trend), and a held-out mean of 0.83.
+Neumann form).  The shipped run's record: mean ratio 1.54 over the first 200
+systems (the untrained readout), medians 0.826, 0.790, 0.783 and 0.768 over the
+four quarters, median 0.754 over the last 200 (single systems can reach ratios
+above 1 at any stage, so medians carry the trend), and a held-out mean of 0.76.
 The test asserts that shape.  It does not assert bit equality with the shipped
 file: the run is bit-reproducible on one build, but its 4000 Adam steps amplify
 last-bit differences in the logged pairs, which any change to the CG kernels'
