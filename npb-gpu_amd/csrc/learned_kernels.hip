@@ -845,16 +845,21 @@ __global__ __launch_bounds__(kBlock) void precond_values_kernel(
     }
 }
 
-// z = M^-1 r, 8 lanes per row; p = z when p is given; r.z partials.  The
-// gathers are latency-bound, so the workgroups are 1024 wide: the partial
-// count stays that of the vector kernels while 4x more rows are in flight.
+// z = M^-1 r, 4 lanes per row; p = z when p is given; r.z partials.  A row
+// holds the diagonal, 4 out-neighbours and its in-neighbours (9 entries on
+// average), so a lane's entries e, e + 4, e + 8 are loaded together and one
+// round of dependent loads (row bounds, then columns and values, then the
+// gathered r) serves the whole row.  The gathers are latency-bound, so the
+// workgroups are 1024 wide and the vector kernels' grid (the partial count
+// the next kernel expects) holds every row of n = 150000 in one pass.
 constexpr int kApplyBlock = 1024;
 template <typename T>
 __global__ __launch_bounds__(kApplyBlock) void precond_apply_kernel(
     int n, const int *__restrict__ off, const int *__restrict__ col, const T *__restrict__ val,
     const T *__restrict__ r, T *__restrict__ z, T *__restrict__ p, double *__restrict__ rz_part,
     const Scalars *__restrict__ sc) {
-    constexpr int kLanes = 8;
+    constexpr int kLanes = 4;
+    constexpr int kUnroll = 3;
     constexpr int kRows = kApplyBlock / kLanes;
     if (sc && sc->done) return;
     const int sub = threadIdx.x % kLanes;
@@ -864,7 +869,18 @@ __global__ __launch_bounds__(kApplyBlock) void precond_apply_kernel(
          j += stride) {
         T zj = 0;
         const int e1 = off[j + 1];
-        for (int e = off[j] + sub; e < e1; e += kLanes) zj += val[e] * r[col[e]];
+        for (int e = off[j] + sub; e < e1; e += kLanes * kUnroll) {
+            T v[kUnroll];
+            int c[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const bool in = e + u * kLanes < e1;
+                v[u] = in ? val[e + u * kLanes] : T(0);
+                c[u] = in ? col[e + u * kLanes] : (int)j;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) zj += v[u] * r[c[u]];
+        }
 #pragma unroll
         for (int m = kLanes / 2; m >= 1; m >>= 1) zj += __shfl_xor(zj, m, kWave);
         if (sub == 0) {
