@@ -335,10 +335,12 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
     const double *__restrict__ rows_part, const int *__restrict__ row_slot, int nblk,
     Scalars *sc, int first_step, const double *__restrict__ minv,
     const double *__restrict__ tol_part) {
-    if (sc->done) return;
-    // This thread's first row (its block partials of q, p, z, r) is loaded
-    // before the scalar reductions, which it does not depend on, so the two
-    // memory round trips overlap; the sums keep their order.
+    // The solve's stop flag is read with the prefetch below and tested after
+    // it (no kernel of this launch writes it), so its round trip overlaps
+    // the others.  This thread's first row (its block partials of q, p, z, r)
+    // is loaded before the scalar reductions, which it does not depend on, so
+    // the memory round trips overlap; the sums keep their order.
+    const int done = sc->done;
     const int j0 = xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     constexpr int kPre = 8;  // block partials of q prefetched
     double qb[kPre], p0 = 0.0, z0 = 0.0, r0 = 0.0, m0 = 0.0;
@@ -359,6 +361,7 @@ __global__ __launch_bounds__(kBlock) void update_zr_kernel(
         r0 = r[j0];
         if (minv) m0 = minv[j0];
     }
+    if (done) return;  // the same value for the whole grid
     double rho, d;
     reduce_partials2<kBlock>(rho_part, n_rho, pq_part, n_pq, rho, d);
     const double alpha = rho / d;
@@ -485,12 +488,17 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
         z0 = z[j0];
         r0 = r[j0];
     }
+    // the previous step's scalars (slot (step - 1) & 1; workgroup 0 writes
+    // slot step & 1 below) are loaded before the reductions as well
+    double rho_prev = 1.0, alpha_prev = 1.0;
+    if (step > 0) {
+        rho_prev = sc->cg_rho[(step - 1) & 1];
+        alpha_prev = sc->cg_alpha[(step - 1) & 1];
+    }
     double rho, rs;
     reduce_partials2<kBlock>(rho_part, n_rho, rs_part, n_rs, rho, rs);
     double beta = 0.0, delta = rs;
     if (step > 0) {
-        const double rho_prev = sc->cg_rho[(step - 1) & 1];
-        const double alpha_prev = sc->cg_alpha[(step - 1) & 1];
         beta = rho / rho_prev;
         delta = rs - beta * rho / alpha_prev;
     }

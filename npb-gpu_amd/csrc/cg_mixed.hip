@@ -200,10 +200,12 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
     const double *__restrict__ pq_part, int n_pq, double *__restrict__ rr_out,
     const float *__restrict__ rows_part, int nblk, Scalars *sc, int first_step,
     const float *__restrict__ minv, const double *__restrict__ tol_part) {
-    if (sc->done) return;
-    // This thread's first row (block partials of q, p, d, r) is loaded before
-    // the scalar reductions, which it does not depend on, so the memory round
-    // trips overlap; the sums keep their order.
+    // The stop flag is read with the prefetch below and tested after it (no
+    // kernel of this launch writes it).  This thread's first row (block
+    // partials of q, p, d, r) is loaded before the scalar reductions, which it
+    // does not depend on, so the memory round trips overlap; the sums keep
+    // their order.
+    const int done = sc->done;
     const int j0 = blockIdx.x * kBlock + threadIdx.x;
     constexpr int kPre = 8;  // block partials of q prefetched
     float qb[kPre], p0 = 0.0f, d0 = 0.0f, r0 = 0.0f, m0 = 0.0f;
@@ -216,6 +218,7 @@ __global__ __launch_bounds__(kBlock) void update_zr32_kernel(
         r0 = r[j0];
         if (minv) m0 = minv[j0];
     }
+    if (done) return;  // the same value for the whole grid
     double rho, dd;
     reduce_partials2<kBlock>(rho_part, gridDim.x, pq_part, n_pq, rho, dd);
     const double alpha = rho / dd;
