@@ -187,32 +187,50 @@ __global__ __launch_bounds__(kBlock) void indegree_kernel(const int *__restrict_
         if (wgt[e] != 0.0f) atomicAdd(&cnt[nbr[e]], 1);
 }
 
-// Exclusive prefix sum of cnt[0..n) into ptr[0..n] (one 1024-thread block):
-// per-thread chunk sums, a Hillis-Steele scan of the 1024 chunk sums in LDS,
-// then each thread writes its chunk.
+// ptr[j] = sum of cnt[0..j) for j <= n (exact, so any order gives the same
+// values): one 1024-thread workgroup walks the counts in tiles of 8192, each
+// thread taking 8 consecutive counts (coalesced across the wavefront), a
+// wavefront scan of the thread sums, then the wavefront totals through LDS,
+// with a running carry between tiles.
 __global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ cnt,
                                                     int *__restrict__ ptr, int n) {
-    __shared__ int part[2][1024];
-    const int t = threadIdx.x;
-    const int chunk = (n + 1023) / 1024;
-    const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
-    int s = 0;
-    for (int i = lo; i < hi; i++) s += cnt[i];
-    part[0][t] = s;
-    __syncthreads();
-    int cur = 0;
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = part[cur][t] + (t >= off ? part[cur][t - off] : 0);
-        part[cur ^ 1][t] = v;
-        cur ^= 1;
+    constexpr int kPer = 8, kTile = 1024 * kPer, kWaves = 1024 / kWave;
+    __shared__ int wsum[kWaves];
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    int carry = 0;
+    for (int base = 0; base < n; base += kTile) {
+        int v[kPer], s = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int i = base + t * kPer + k;
+            v[k] = i < n ? cnt[i] : 0;
+            s += v[k];
+        }
+        int x = s;  // inclusive scan over the wavefront
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int y = __shfl_up(x, off, kWave);
+            if (lane >= off) x += y;
+        }
+        if (lane == kWave - 1) wsum[w] = x;
         __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int i = 0; i < kWaves; i++) {
+            if (i < w) before += wsum[i];
+            total += wsum[i];
+        }
+        int e = carry + before + x - s;  // exclusive prefix of this thread's first count
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int i = base + t * kPer + k;
+            if (i < n) ptr[i] = e;
+            e += v[k];
+        }
+        carry += total;
+        __syncthreads();  // wsum is rewritten by the next tile
     }
-    int acc = part[cur][t] - s;  // exclusive
-    if (t == 1023) ptr[n] = part[cur][t];
-    for (int i = lo; i < hi; i++) {
-        ptr[i] = acc;
-        acc += cnt[i];
-    }
+    if (t == 0) ptr[n] = carry;
 }
 
 __global__ __launch_bounds__(kBlock) void transpose_fill_kernel(
