@@ -471,7 +471,8 @@ def batch_comparison(reps: int):
                      steps_median=float(np.median(steps)),
                      unconverged_systems=sorted({i + 1 for i in np.nonzero(rel > bw["rtol"])[0]}),
                      tuned_spmm_ms=round(ms, 4),
-                     tuned_spmm_gbs=round(spmm_bytes / (ms * 1e-3) / 1e9, 1))
+                     tuned_spmm_gbs=round(spmm_bytes / (ms * 1e-3) / 1e9, 1),
+                     tuned_spmm_frac=round(spmm_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
             out[name] = d
     return out
 
