@@ -75,7 +75,11 @@ typedef struct npbcg_opts {
                          Solves always use the reference's step order. */
     int matrix_check; /* npbcg_conj_grad / npbcg_compat_conj_grad: how the host arrays
                          are checked against the context's matrix (NPBCG_CHECK_*) */
-    int reserved[5];
+    int dot_pass;     /* one-reduction form: where r.s is formed.  0 = auto (in its own
+                         pass from 24 column blocks up), 1 = in the SpMV, 2 = in its own
+                         pass (the SpMV stores the block sums of s = A r only; a combine
+                         kernel adds them into s and forms r.s; three launches a step) */
+    int reserved[4];
 } npbcg_opts;
 
 /* Check of the host arrays handed to npbcg_conj_grad (npbcg_opts.matrix_check). */

@@ -164,6 +164,9 @@ struct npbcg_ctx {
 // Device vectors: x, z, p, q, r of the reference plus p_alt (fused p
 // update), b and the solution of npbcg_solve.
 constexpr int kNumVectors = 8;
+// Column blocks from which the one-reduction form forms r.s in its own pass
+// (npbcg_opts.dot_pass = 0).
+constexpr int kDotApartBlocks = 24;
 constexpr int kVecPAlt = 5, kVecB = 6, kVecXsol = 7;
 
 namespace {
@@ -519,6 +522,10 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
         delete c;
         return set_error(NPBCG_ERR_INVALID, "npbcg_create: unknown engine");
     }
+    if (c->opts.dot_pass < 0 || c->opts.dot_pass > 2) {
+        delete c;
+        return set_error(NPBCG_ERR_INVALID, "npbcg_create: dot_pass must be 0, 1 or 2");
+    }
     // AUTO takes the graph-replayed kernels: 8.5 ms against 14.6 ms for the
     // persistent kernel over 10 outer iterations at n = 75000, whose grid
     // barriers each write back and invalidate every XCD's L2
@@ -695,6 +702,11 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
     // partials) and no fused p update
     c->pb.cg_form = (variant == npbcg_dev::kSpmvTiled && !c->pb.fuse_p_update &&
                      c->opts.cg_form != 1) ? 1 : 0;
+    // r.s in a pass of its own once each row has only a few entries per
+    // column block (NPB class D: 85 blocks of about 5 entries per row)
+    c->pb.dot_apart = c->pb.cg_form && (c->opts.dot_pass == 2 ||
+                                        (c->opts.dot_pass == 0 && sd.nblk >= kDotApartBlocks))
+                          ? 1 : 0;
     c->pb.sell = sd;
     refresh_solve_view(c);
     if (c->engine == NPBCG_ENGINE_PERSISTENT) {
@@ -1079,11 +1091,12 @@ int npbcg_describe(const npbcg_ctx *c, char *buf, int len) {
         "\"csr_blocks\": %d, \"block_width\": %d, \"col_blocks\": %d, "
         "\"range_rows\": %d, \"row_ranges\": %d, \"sell_workgroups\": %d, "
         "\"sell_threads\": %d, \"sell_stored\": %lld, \"cgitmax\": %d, "
-        "\"use_graph\": %d, \"fuse_p_update\": %d, \"cg_form\": %d, \"engine\": %d, "
-        "\"device_name\": \"%s\", \"compute_units\": %d}",
+        "\"use_graph\": %d, \"fuse_p_update\": %d, \"cg_form\": %d, \"dot_pass\": %d, "
+        "\"engine\": %d, \"device_name\": \"%s\", \"compute_units\": %d}",
         c->n, c->nnz, c->pb.spmv_variant, c->pb.vec_blocks, c->pb.spmv_blocks, sd.W,
         sd.nblk, sd.R, sd.nrange, sd.grid, sd.block, c->sell_stored, c->cgitmax,
-        c->opts.use_graph, c->pb.fuse_p_update, c->pb.cg_form ? 2 : 1, c->engine, prop.name,
+        c->opts.use_graph, c->pb.fuse_p_update, c->pb.cg_form ? 2 : 1,
+        c->pb.dot_apart ? 2 : 1, c->engine, prop.name,
         prop.multiProcessorCount);
     if (len > 0) {
         std::strncpy(buf, tmp, (size_t)len - 1);

@@ -471,18 +471,23 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
     const double *__restrict__ rs_part, int n_rs, double *__restrict__ rr_out,
     const double *__restrict__ rows_part, const int *__restrict__ row_slot, int nblk,
     Scalars *sc, int step) {
-    // this thread's first row, loaded ahead of the reductions
+    // this thread's first row, loaded ahead of the reductions.  Without
+    // row_slot, rows_part holds s itself (nblk = 1, row j at j).
     const int j0 = xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     constexpr int kPre = 8;
     double sb[kPre], p0 = 0.0, q0 = 0.0, z0 = 0.0, r0 = 0.0;
     if (j0 < n) {
-        int sl[kPre];
+        if (row_slot) {
+            int sl[kPre];
 #pragma unroll
-        for (int b = 0; b < kPre; b++)
-            if (b == 0 || b < nblk) sl[b] = row_slot[(size_t)b * n + j0];
+            for (int b = 0; b < kPre; b++)
+                if (b == 0 || b < nblk) sl[b] = row_slot[(size_t)b * n + j0];
 #pragma unroll
-        for (int b = 0; b < kPre; b++)
-            if (b == 0 || b < nblk) sb[b] = rows_part[sl[b]];
+            for (int b = 0; b < kPre; b++)
+                if (b == 0 || b < nblk) sb[b] = rows_part[sl[b]];
+        } else {
+            sb[0] = rows_part[j0];
+        }
         p0 = p[j0];
         q0 = q[j0];
         z0 = z[j0];
@@ -526,6 +531,12 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
             qj = q0;
             zj = z0;
             rj = r0;
+        } else if (!row_slot) {
+            sj = rows_part[j];
+            pj = p[j];
+            qj = q[j];
+            zj = z[j];
+            rj = r[j];
         } else {
             sj = rows_part[row_slot[j]];
 #pragma unroll 8
@@ -545,6 +556,27 @@ __global__ __launch_bounds__(kBlock) void update_cg_kernel(
         acc += rn * rn;
     }
     store_block_partial<kBlock>(acc, rr_out);
+}
+
+// s = sum of the column blocks' row sums of A r (blocks in order, each found
+// at its slot, the same sum update_cg forms), stored; partials of r.s.  Used
+// when the SpMV leaves out r.s: with many column blocks each slice holds only
+// a few entries per row, and gathering r[row] for the dot in every block cost
+// more than this pass.
+__global__ __launch_bounds__(kBlock) void combine_rs_kernel(
+    const double *__restrict__ r, const double *__restrict__ rows_part,
+    const int *__restrict__ row_slot, int nblk, int n, double *__restrict__ s_out,
+    double *__restrict__ rs_out) {
+    double acc = 0.0;
+    for (int j = xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < n;
+         j += gridDim.x * kBlock) {
+        double sj = rows_part[row_slot[j]];
+#pragma unroll 8
+        for (int b = 1; b < nblk; b++) sj += rows_part[row_slot[(size_t)b * n + j]];
+        s_out[j] = sj;
+        acc += r[j] * sj;
+    }
+    store_block_partial<kBlock>(acc, rs_out);
 }
 
 // Start of conj_grad (cg.cu:671-686): q = z = 0, r = p = x for j <= n and
@@ -786,6 +818,16 @@ hipError_t launch_sell_t(const Fp64Problem &pb, int step, hipStream_t st, hipEve
                               rev),
                      "spmv_sell_kernel (p = r + beta p; q = A p)", BLOCK, st);
     }
+    if (MODE == 0 && pb.cg_form && pb.dot_apart) {  // s = A r, block sums only
+        return noted(pb,
+                     launch_k(spmv_sell_kernel<BLOCK, kSellUnr, 1, false>, d.grid, BLOCK,
+                              d.lds_bytes, st, e0, e1, d.wg_slice, d.slice_off, d.slice_row,
+                              d.val, d.col, (const double *)pb.r, d.rows_part, pb.part.pq, pb.n,
+                              d.W, d.nrange, (const double *)nullptr, (double *)nullptr,
+                              (const double *)nullptr, (const double *)nullptr, 0,
+                              (const Scalars *)pb.sc, step & 1),
+                     "spmv_sell_kernel (s = A r, r.s apart)", BLOCK, st);
+    }
     // one-reduction form: the CG SpMV is s = A r (its dot partials are r.s)
     const double *v = MODE == 0 ? (pb.cg_form ? pb.r : p_of_step(pb, step)) : pb.z;
     return noted(pb,
@@ -980,10 +1022,21 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
             e = launch_spmv<0>(pb, st, timed ? probe->start[it] : nullptr,
                                timed ? probe->stop[it] : nullptr, it);
             if (e != hipSuccess) break;
-            update_cg_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
-                pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, it), pb.vec_blocks, pb.part.pq,
-                spmv_grid(pb), pb.part.rr[it & 1], pb.sell.rows_part, pb.sell.row_slot,
-                pb.sell.nblk, pb.sc, it);
+            if (pb.dot_apart) {  // s into p_alt with its r.s partials, then the update
+                combine_rs_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+                    pb.r, pb.sell.rows_part, pb.sell.row_slot, pb.sell.nblk, pb.n, pb.p_alt,
+                    pb.part.pq);
+                e = noted(pb, hipGetLastError(), "combine_rs_kernel", kBlock, st);
+                if (e != hipSuccess) break;
+                update_cg_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+                    pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, it), pb.vec_blocks, pb.part.pq,
+                    pb.vec_blocks, pb.part.rr[it & 1], pb.p_alt, nullptr, 1, pb.sc, it);
+            } else {
+                update_cg_kernel<<<pb.vec_blocks, kBlock, 0, st>>>(
+                    pb.p, pb.q, pb.z, pb.r, pb.n, rho_source(pb, it), pb.vec_blocks, pb.part.pq,
+                    spmv_grid(pb), pb.part.rr[it & 1], pb.sell.rows_part, pb.sell.row_slot,
+                    pb.sell.nblk, pb.sc, it);
+            }
             e = noted(pb, hipGetLastError(), "update_cg_kernel", kBlock, st);
         }
         if (e == hipSuccess && cgitmax > 0) e = launch_update_p(pb, cgitmax - 1, st);
@@ -1007,7 +1060,7 @@ hipError_t enqueue_conj_grad(const Fp64Problem &pb, int cgitmax, int scale_x, in
 
 int conj_grad_launches(const Fp64Problem &pb, int cgitmax) {
     const int resid = pb.spmv_variant == kSpmvTiled ? 3 : 2;
-    if (pb.cg_form) return 1 + 2 * cgitmax + (cgitmax > 0 ? 1 : 0) + resid;
+    if (pb.cg_form) return 1 + (pb.dot_apart ? 3 : 2) * cgitmax + (cgitmax > 0 ? 1 : 0) + resid;
     if (fused_p(pb)) return 1 + 2 * cgitmax + (cgitmax > 0 ? 1 : 0) + resid;
     return 1 + 3 * cgitmax + resid;
 }

@@ -84,6 +84,9 @@ struct Fp64Problem {
     int fuse_p_update;          // SELL path: p update folded into the next SpMV
     int cg_form;                // SELL path, conj_grad only: 1 = one-reduction form
                                 // (s = A r, then one update kernel per step)
+    int dot_apart;              // with cg_form: the SpMV stores s = A r without r.s;
+                                // combine_rs adds the block sums into s (in p_alt)
+                                // and forms r.s, then update_cg reads s directly
     const double *minv;         // diagonal preconditioner M^-1 (nullptr: plain CG)
     const PrecondOp *pop;       // sparse preconditioner (nullptr: none); z = M^-1 r in pz
     double *pz;

@@ -50,7 +50,8 @@ class Opts(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("tile_rows", ctypes.c_int),
                 ("tile_width", ctypes.c_int), ("fuse_p_update", ctypes.c_int),
                 ("solve_batch", ctypes.c_int), ("cg_form", ctypes.c_int),
-                ("matrix_check", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("matrix_check", ctypes.c_int), ("dot_pass", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 SOLVE_FP64, SOLVE_MIXED = 0, 1

@@ -250,24 +250,29 @@ def test_single_system_tuner_persists_and_matches_oracle(gpu, golden, tmp_path, 
     _zeta_close(zeta, ozeta, f"{size} tuned shape vs oracle")
 
 
-@pytest.mark.parametrize("cg_form", [1, 2])
+@pytest.mark.parametrize("cg_form,dot_pass", [(1, 0), (2, 1), (2, 2)])
 @pytest.mark.parametrize("name", ["S_npb", "A_npb", "B_npb", "cfg3_n150000_seed314"])
-def test_cg_forms_match_reference(gpu, golden, name, cg_form):
+def test_cg_forms_match_reference(gpu, golden, name, cg_form, dot_pass):
     # cg_form 1: the reference's step order; 2: the one-reduction form (s = A r,
-    # p.q from r.s and the previous step's scalars).  Both are held to the
-    # reference's own zeta and rnorm trajectories.
+    # p.q from r.s and the previous step's scalars), with r.s formed in the
+    # SpMV (dot_pass 1) or in a combine pass of its own (dot_pass 2, the
+    # default from 24 column blocks up).  All are held to the reference's own
+    # zeta and rnorm trajectories.
     c = golden[name]
     m = _gen(gpu, c)
-    with gpu.Context(m, spmv_variant=5, cg_form=cg_form) as ctx:
-        assert ctx.describe()["cg_form"] == cg_form
+    with gpu.Context(m, spmv_variant=5, cg_form=cg_form, dot_pass=dot_pass) as ctx:
+        d = ctx.describe()
+        assert d["cg_form"] == cg_form
+        assert d["dot_pass"] == (dot_pass if cg_form == 2 else 1)
         zeta, rnorm = ctx.power(c["niter"], c["shift"])
-    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"]], f"{name} form {cg_form}")
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"]],
+                f"{name} form {cg_form} dot pass {dot_pass}")
     rref = np.array([float.fromhex(h) for h in c["rnorm_hex"]])
     assert np.all(np.abs(rnorm - rref) <= 1e-12 * np.sqrt(m.n))
 
 
-@pytest.mark.parametrize("cg_form", [1, 2])
-def test_cg_forms_dropin_vectors(gpu, golden, cg_form):
+@pytest.mark.parametrize("cg_form,dot_pass", [(1, 0), (2, 1), (2, 2)])
+def test_cg_forms_dropin_vectors(gpu, golden, cg_form, dot_pass):
     c = golden["cfg1_n1400_seed12345"]
     m = _gen(gpu, c)
     n = m.n
@@ -275,7 +280,7 @@ def test_cg_forms_dropin_vectors(gpu, golden, cg_form):
     x = np.ones(n + 2)
     x[:n] += 0.25 * rng.standard_normal(n)
     x[n] = 2.0
-    with gpu.Context(m, spmv_variant=5, cg_form=cg_form) as ctx:
+    with gpu.Context(m, spmv_variant=5, cg_form=cg_form, dot_pass=dot_pass) as ctx:
         z, p, q, r, rn = ctx.conj_grad(x)
     oz, op, oq, orr, orn = oracle.conj_grad(_ocsr(m), x)
     xmax = float(np.max(np.abs(x[:n])))

@@ -117,3 +117,6 @@ def test_persistent_needs_the_sell_layout(gpu):
     with pytest.raises(gpu.NpbcgError) as e:
         gpu.Context(m, engine=7)
     assert e.value.code == gpu.ERR_INVALID
+    with pytest.raises(gpu.NpbcgError) as e:
+        gpu.Context(m, dot_pass=3)
+    assert e.value.code == gpu.ERR_INVALID
