@@ -85,3 +85,36 @@ def test_null_context_calls_are_errors(npbcg_lib):
     assert L.npbcg_profile_kernels(None, 1, 10.0, np.zeros(1), kt, 4, ctypes.byref(nk),
                                    ctypes.byref(d)) == npbcg_lib.ERR_INVALID
     L.npbcg_destroy(None)
+
+
+def test_python_structs_match_the_header(npbcg_lib, tmp_path):
+    # Every struct the binding passes by pointer has the header's size and
+    # field offsets (compiled with the host C compiler against include/npbcg.h).
+    import os
+    pairs = [("npbcg_opts", npbcg_lib.Opts), ("npbcg_timing", npbcg_lib.Timing),
+             ("npbcg_kernel_time", npbcg_lib.KernelTime),
+             ("npbcg_solve_stats", npbcg_lib.SolveStats),
+             ("npbcg_train_opts", npbcg_lib.TrainOpts), ("npbcg_train_stats", npbcg_lib.TrainStats),
+             ("npbcg_batch_opts", npbcg_lib.BatchOpts), ("npbcg_batch_stats", npbcg_lib.BatchStats)]
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "npbcg.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'    printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line.strip():
+            name, field, val = line.split()
+            got[(name, field)] = int(val)
+    for cname, py in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
+
