@@ -76,7 +76,11 @@ __device__ __forceinline__ double safe_abs(double d) { return d != 0.0 ? fabs(d)
 // One wavefront per row: the features and the top-K neighbours.  Selection
 // round t takes the best candidate (|a| descending, column ascending) strictly
 // after round t-1's; a row with fewer than K off-diagonals is padded with
-// zero-weight self edges.
+// zero-weight self edges.  A row of at most kFeatCache * 64 entries is loaded
+// into registers once (lane l holds entries l, l + 64, ...; absent entries
+// carry column j, which every pass skips as it skips the diagonal) and the
+// feature pass and the K selection rounds read it from there, in the same
+// order as from memory; longer rows are read from memory in every pass.
 __global__ __launch_bounds__(kBlock) void graph_features_kernel(
     const int *__restrict__ rowstr, const int *__restrict__ colidx,
     const double *__restrict__ a, const double *__restrict__ adiag, int n,
@@ -84,23 +88,46 @@ __global__ __launch_bounds__(kBlock) void graph_features_kernel(
     float *__restrict__ nval) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    constexpr int kFeatCache = 8;
     const int nw = gridDim.x * kWavesPerBlock;
     for (int j = blockIdx.x * kWavesPerBlock + wave; j < n; j += nw) {
         const int s = rowstr[j], e = rowstr[j + 1];
         const double dj = adiag[j];
         const double aj = safe_abs(dj);
+        const bool cached = e - s <= kFeatCache * kWave;  // the same for the whole wavefront
+        int cc[kFeatCache];
+        double cv[kFeatCache], cd[kFeatCache];
+        if (cached) {
+#pragma unroll
+            for (int u = 0; u < kFeatCache; u++) {
+                const int k = s + lane + u * kWave;
+                cc[u] = k < e ? colidx[k] : j;
+                cv[u] = k < e ? a[k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kFeatCache; u++) cd[u] = cc[u] != j ? adiag[cc[u]] : 1.0;
+        }
         double s_abs = 0, s_sq = 0, s_sgn = 0, s_nb = 0, s_w = 0, mx = 0;
-        for (int k = s + lane; k < e; k += kWave) {
-            const int c = colidx[k];
-            if (c == j) continue;
-            const double v = a[k], av = fabs(v);
-            const double ak = safe_abs(adiag[c]);
+        auto feature_terms = [&](int c, double v, double adc) {
+            const double av = fabs(v);
+            const double ak = safe_abs(adc);
             s_abs += av;
             s_sq += v * v;
             s_sgn += v;
             s_nb += v * v / (aj * ak);
             s_w += av / sqrt(aj * ak);
             mx = fmax(mx, av);
+        };
+        if (cached) {
+#pragma unroll
+            for (int u = 0; u < kFeatCache; u++)
+                if (cc[u] != j) feature_terms(cc[u], cv[u], cd[u]);
+        } else {
+            for (int k = s + lane; k < e; k += kWave) {
+                const int c = colidx[k];
+                if (c == j) continue;
+                feature_terms(c, a[k], adiag[c]);
+            }
         }
         s_abs = segment_sum<64>(s_abs);
         s_sq = segment_sum<64>(s_sq);
@@ -128,16 +155,25 @@ __global__ __launch_bounds__(kBlock) void graph_features_kernel(
         for (int t = 0; t < kNbr; t++) {
             double bv = -1.0, bval = 0.0;
             int bc = 0x7fffffff;
-            for (int k = s + lane; k < e; k += kWave) {
-                const int c = colidx[k];
-                if (c == j) continue;
-                const double v = a[k], av = fabs(v);
+            auto candidate = [&](int c, double v) {
+                const double av = fabs(v);
                 const bool after = av < pv || (av == pv && c > pc);
                 const bool better = av > bv || (av == bv && c < bc);
                 if (after && better) {
                     bv = av;
                     bc = c;
                     bval = v;
+                }
+            };
+            if (cached) {
+#pragma unroll
+                for (int u = 0; u < kFeatCache; u++)
+                    if (cc[u] != j) candidate(cc[u], cv[u]);
+            } else {
+                for (int k = s + lane; k < e; k += kWave) {
+                    const int c = colidx[k];
+                    if (c == j) continue;
+                    candidate(c, a[k]);
                 }
             }
             for (int m = 32; m >= 1; m >>= 1) {
