@@ -79,7 +79,12 @@ typedef struct npbcg_opts {
                          pass from 24 column blocks up), 1 = in the SpMV, 2 = in its own
                          pass (the SpMV stores the block sums of s = A r only; a combine
                          kernel adds them into s and forms r.s; three launches a step) */
-    int reserved[4];
+    int sort_window;  /* LDS-staged SpMV: rows of a workgroup's range sorted together by
+                         their length in a column block before they are cut into 64-row
+                         slices (a multiple of 64).  0 = auto: the whole range, within
+                         2048 .. 8192 rows.  Wider windows pad fewer entries; narrower
+                         ones keep a slice's block sums closer together. */
+    int reserved[3];
 } npbcg_opts;
 
 /* Check of the host arrays handed to npbcg_conj_grad (npbcg_opts.matrix_check). */

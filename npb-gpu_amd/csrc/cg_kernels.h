@@ -180,6 +180,7 @@ struct MixedProblem {
     double *part_pq = nullptr;    // p.q partials [sell grid]
     Scalars *sc = nullptr;        // inner-solve scalars
     int lds_bytes = 0;
+    int stage_rows = 0;           // rows per range collected in LDS by the SpMV (0: none)
 };
 hipError_t prepare_mixed_kernels(int lds_bytes);
 // The FP32 SELL SpMV kernel of an inner CG step (q = A p with p.q partials,

@@ -32,6 +32,12 @@ __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double *__restric
 
 // q = A p in FP32 over the SELL layout (see spmv_sell_kernel): per-block row
 // sums into rows32[b][row], FP64 partials of p.q.
+// stage_rows = R (rows per range) collects the sums of the workgroup's rows,
+// all of row range wg % nrange, in LDS and stores them once the slices are
+// done, as one coalesced run by row; a slice's 64 rows lie anywhere in their
+// sort window, so storing them straight away writes 4 bytes into each of up
+// to 64 cache lines.  stage_rows = 0 stores them directly (ranges whose rows
+// do not fit in the LDS beside the vector block).
 // PUPD: the previous step's p update is done here.  beta = (r.r) / (old r.r)
 // from the partials and the stop test on r.r (one workgroup records it; all
 // workgroups reach the same decision and return); else p = r + beta p_old is
@@ -45,7 +51,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     float *__restrict__ rows_part, double *__restrict__ dot_part, int n, int W, int nrange,
     Scalars *sc, const float *__restrict__ r, float *__restrict__ p_new,
     const double *__restrict__ rr_part, const double *__restrict__ rho_part, int n_vec,
-    int rev) {
+    int rev, int stage_rows) {
     if (sc->done) return;
     float b32 = 0.0f;
     if constexpr (PUPD) {
@@ -66,12 +72,14 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     constexpr int kWaves = BLOCK / kWave;
     const int wg = blockIdx.x;
     const int b = wg / nrange;
+    const int r0 = (wg % nrange) * stage_rows;  // first row of the range (staged)
     const int c0 = b * W;
     const int wb = min(W, n - c0);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int s0 = wg_slice[wg];
     const int nsl = wg_slice[wg + 1] - s0;
+    float *rows_lds = reinterpret_cast<float *>(slice_dot + nsl);  // stage_rows entries
     // Slice of round rd for this wave (snake order, widest first), or -1;
     // the index grows with rd, so after the first -1 there is none.
     auto slice_of = [&](int rd) {
@@ -160,7 +168,12 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
             nrow = slice_row[(size_t)(s0 + nxt) * kWave + lane];
             if (xwidth > 0) sell_chunk<float, UNR>(val, col, xbase, xwidth, 0, lane, na, nc);
         }
-        if (row >= 0) part_b[row] = acc;
+        if (row >= 0) {
+            if (stage_rows)
+                rows_lds[row - r0] = acc;
+            else
+                part_b[row] = acc;
+        }
         const double d = segment_sum<64>((double)pr * (double)acc);
         if (lane == 0) slice_dot[k] = d;
         cur = nxt;
@@ -168,6 +181,11 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
         cwidth = xwidth;
     }
     __syncthreads();
+    if (stage_rows) {
+        // every row of the range has a slot in each block, so all are set
+        const int nr = min(stage_rows, n - r0);
+        for (int i = threadIdx.x; i < nr; i += BLOCK) part_b[r0 + i] = rows_lds[i];
+    }
     double dot = 0.0;
     for (int i = threadIdx.x; i < nsl; i += BLOCK) dot += slice_dot[i];
     store_block_partial<BLOCK>(dot, dot_part);
@@ -390,11 +408,13 @@ static hipError_t launch_spmv32(const Fp64Problem &pb, const MixedProblem &mp, c
     if (sd.block == 512)
         spmv_sell32_kernel<512, kSellUnr32, PUPD><<<sd.grid, 512, mp.lds_bytes, st>>>(
             sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
-            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev);
+            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev,
+            mp.stage_rows);
     else
         spmv_sell32_kernel<1024, kSellUnr32, PUPD><<<sd.grid, 1024, mp.lds_bytes, st>>>(
             sd.wg_slice, sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq,
-            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev);
+            pb.n, sd.W, sd.nrange, mp.sc, mp.r, p_new, rr_part, rho_part, pb.vec_blocks, rev,
+            mp.stage_rows);
     return hipGetLastError();
 }
 
@@ -410,13 +430,14 @@ hipError_t launch_spmv32_kernel(const Fp64Problem &pb, const MixedProblem &mp, i
         hipExtLaunchKernelGGL(spmv_sell32_kernel<512, kSellUnr32, false>, dim3(sd.grid), dim3(512),
                               (uint32_t)mp.lds_bytes, st, e0, e1, 0u, sd.wg_slice, sd.slice_off,
                               sd.slice_row, mp.val, sd.col, v, mp.rows_part, mp.part_pq, pb.n, sd.W,
-                              sd.nrange, mp.sc, r_none, p_none, d_none, d_none, pb.vec_blocks, rev);
+                              sd.nrange, mp.sc, r_none, p_none, d_none, d_none, pb.vec_blocks, rev,
+                              mp.stage_rows);
     else
         hipExtLaunchKernelGGL(spmv_sell32_kernel<1024, kSellUnr32, false>, dim3(sd.grid),
                               dim3(1024), (uint32_t)mp.lds_bytes, st, e0, e1, 0u, sd.wg_slice,
                               sd.slice_off, sd.slice_row, mp.val, sd.col, v, mp.rows_part,
                               mp.part_pq, pb.n, sd.W, sd.nrange, mp.sc, r_none, p_none, d_none,
-                              d_none, pb.vec_blocks, rev);
+                              d_none, pb.vec_blocks, rev, mp.stage_rows);
     return hipGetLastError();
 }
 

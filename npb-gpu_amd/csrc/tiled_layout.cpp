@@ -26,19 +26,19 @@ void parallel_items(int count, F &&body) {
 }
 
 // Rows of range rr ordered by their segment length in block b (longest
-// first, ties by row) within windows of kSortWindow rows, with the segment
+// first, ties by row) within windows of `window` rows, with the segment
 // bounds.  The window keeps the rows of one slice, whose block sums are
 // stored as 8-byte partials, within a few thousand rows of each other, so
-// the partial stores of large ranges stay in L2-sized neighbourhoods.  It
-// is a multiple of the slice height, so a slice never spans two windows.
+// the partial reads of the combining kernels (row by row, through row_slot)
+// stay in L2-sized neighbourhoods; a wider window pads fewer entries (rows of
+// like length share a slice).  It is a multiple of the slice height, so a
+// slice never spans two windows.
 struct RowSeg {
     int row, lo, hi;
 };
-constexpr int kSortWindow = 2048;
-static_assert(kSortWindow % kSliceRows == 0, "window of whole slices");
 
 void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, int b,
-                    int rr, std::vector<RowSeg> *out) {
+                    int rr, int window, std::vector<RowSeg> *out) {
     out->clear();
     const int r0 = rr * R, r1 = std::min(n, r0 + R);
     const int c0 = b * W, c1 = c0 + W;
@@ -48,8 +48,8 @@ void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, i
         const int hi = (int)(std::lower_bound(first, last, c1) - colidx);
         out->push_back({j, lo, hi});
     }
-    for (size_t w0 = 0; w0 < out->size(); w0 += kSortWindow) {
-        const size_t w1 = std::min(out->size(), w0 + (size_t)kSortWindow);
+    for (size_t w0 = 0; w0 < out->size(); w0 += window) {
+        const size_t w1 = std::min(out->size(), w0 + (size_t)window);
         std::stable_sort(out->begin() + w0, out->begin() + w1,
                          [](const RowSeg &x, const RowSeg &y) {
                              return (x.hi - x.lo) > (y.hi - y.lo);
@@ -59,8 +59,9 @@ void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, i
 
 template <class V>
 bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, int W, int R,
-                SellBlocks<V> *out) {
-    if (n < 1 || W < 1 || W > 65536 || R < 1) return false;
+                int window, SellBlocks<V> *out) {
+    if (n < 1 || W < 1 || W > 65536 || R < 1 || window < kSliceRows || window % kSliceRows)
+        return false;
     SellBlocks<V> &m = *out;
     m.n = n;
     m.W = W;
@@ -76,7 +77,7 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
     parallel_items(nwg, [&](int wg) {
         std::vector<RowSeg> segs;
         const int b = wg / m.nrange, rr = wg % m.nrange;
-        block_segments(n, rowstr, colidx, W, R, b, rr, &segs);
+        block_segments(n, rowstr, colidx, W, R, b, rr, window, &segs);
         long long e = 0;
         int s = 0;
         for (size_t i = 0; i < segs.size(); i += kSliceRows, s++)
@@ -104,7 +105,7 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
     parallel_items(nwg, [&](int wg) {
         std::vector<RowSeg> segs;
         const int b = wg / m.nrange, rr = wg % m.nrange;
-        block_segments(n, rowstr, colidx, W, R, b, rr, &segs);
+        block_segments(n, rowstr, colidx, W, R, b, rr, window, &segs);
         long long off = wg_base[wg];
         int s = m.wg_slice[wg];
         for (size_t i0 = 0; i0 < segs.size(); i0 += kSliceRows, s++) {
@@ -131,13 +132,13 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
 }  // namespace
 
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
-                       int R, SellBlocks<double> *out) {
-    return build_impl(n, rowstr, colidx, a, W, R, out);
+                       int R, SellBlocks<double> *out, int window) {
+    return build_impl(n, rowstr, colidx, a, W, R, window, out);
 }
 
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
-                       int R, SellBlocks<float> *out) {
-    return build_impl(n, rowstr, colidx, a, W, R, out);
+                       int R, SellBlocks<float> *out, int window) {
+    return build_impl(n, rowstr, colidx, a, W, R, window, out);
 }
 
 }  // namespace npbcg_detail

@@ -3,10 +3,10 @@
 // Columns are cut into nblk blocks of W columns and rows into nrange ranges
 // of R rows.  Workgroup wg = b * nrange + rr owns the part of row range rr that
 // falls into column block b.  Inside a workgroup the rows are ordered by their
-// segment length in block b (longest first, ties by row) within windows of
-// 2048 rows, and cut into slices of 64 rows; slice s stores its entries
-// column-major: entry j of the slice's lane l sits at slice_off[s] + 64*j + l,
-// and rows shorter than the slice
+// segment length in block b (longest first, ties by row) within sort windows
+// (build_sell_blocks), and cut into slices of 64 rows; slice s stores its
+// entries column-major: entry j of the slice's lane l sits at
+// slice_off[s] + 64*j + l, and rows shorter than the slice
 // width are padded with (value 0, column 0).  So one wavefront streams a slice
 // with fully coalesced loads, one row per lane, and no cross-lane reduction.
 // Column indices are 16-bit offsets inside the block (W <= 65536).
@@ -42,12 +42,17 @@ struct SellBlocks {
     std::vector<uint16_t> col;   // stored
 };
 
+// Rows are sorted by segment length within windows of `window` rows (a
+// multiple of 64; kSortWindow unless the caller widens it).
+constexpr int kSortWindow = 2048;
+
 // Builds the copy; the CSR must have ascending columns within each row.
-// Returns false if W > 65536 or the sizes overflow 32-bit offsets.
+// Returns false if W > 65536, the window is not a positive multiple of 64
+// or the sizes overflow 32-bit offsets.
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
-                       int W, int R, SellBlocks<double> *out);
+                       int W, int R, SellBlocks<double> *out, int window = kSortWindow);
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
-                       int W, int R, SellBlocks<float> *out);
+                       int W, int R, SellBlocks<float> *out, int window = kSortWindow);
 
 }  // namespace npbcg_detail
 

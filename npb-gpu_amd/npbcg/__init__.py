@@ -51,7 +51,7 @@ class Opts(ctypes.Structure):
                 ("tile_width", ctypes.c_int), ("fuse_p_update", ctypes.c_int),
                 ("solve_batch", ctypes.c_int), ("cg_form", ctypes.c_int),
                 ("matrix_check", ctypes.c_int), ("dot_pass", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("sort_window", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 SOLVE_FP64, SOLVE_MIXED = 0, 1

@@ -271,6 +271,56 @@ def test_cg_forms_match_reference(gpu, golden, name, cg_form, dot_pass):
     assert np.all(np.abs(rnorm - rref) <= 1e-12 * np.sqrt(m.n))
 
 
+@pytest.mark.parametrize("window", [64, 2048, 8192])
+def test_sort_windows_match_reference(gpu, golden, window):
+    # Rows are sorted by their length in a column block within sort windows
+    # before they are cut into 64-row slices.  The window changes which rows
+    # share a slice (the padding, and the order of the per-slice p.q
+    # partials), never the order of a row's own sum.  The n = 150000 matrix
+    # has 4688-row ranges, so 64, 2048 and 8192 (the whole range) give three
+    # layouts; each is held to the reference's zeta and rnorm trajectories,
+    # and a mixed solve on it (FP32 row sums collected in LDS, ranges wider
+    # than 2048 rows) to the FP64 true residual.
+    c = golden["cfg3_n150000_seed314"]
+    m = _gen(gpu, c)
+    b = oracle.make_rhs(m.n, 314.0)
+    with gpu.Context(m, spmv_variant=5, sort_window=window) as ctx:
+        d = ctx.describe()
+        zeta, rnorm = ctx.power(c["niter"], c["shift"])
+        ctx.set_rhs(b)
+        st = ctx.solve(1, 1e-10, 200)
+        x = ctx.solution()
+    assert d["sort_window"] == window
+    _zeta_close(zeta, [float.fromhex(h) for h in c["zeta_hex"]], f"sort window {window}")
+    rref = np.array([float.fromhex(h) for h in c["rnorm_hex"]])
+    assert np.all(np.abs(rnorm - rref) <= 1e-12 * np.sqrt(m.n))
+    rel = float(np.linalg.norm(b - oracle.spmv(_ocsr(m), x)) / np.linalg.norm(b))
+    assert st.converged == 1 and rel <= 1e-10, (st.as_dict(), rel)
+
+
+def test_sort_window_padding_and_default(gpu, golden):
+    # Cutting a descending run into consecutive 64-row slices gives the least
+    # padding of any cut into slices of 64, so padding cannot grow as
+    # windows nest: 64 within 2048 within the whole 4688-row range.  The
+    # default sorts the whole range (4736 rows, from 2048 up to 8192).
+    c = golden["cfg3_n150000_seed314"]
+    m = _gen(gpu, c)
+    stored = {}
+    for w in (64, 2048, 8192, 0):
+        with gpu.Context(m, spmv_variant=5, sort_window=w) as ctx:
+            d = ctx.describe()
+        stored[w] = d["sell_stored"]
+        if w == 0:
+            assert d["sort_window"] == 4736 and d["range_rows"] == 4688
+    print(stored)
+    assert stored[64] >= stored[2048] >= stored[8192] >= m.nnz
+    assert stored[2048] > stored[8192] and stored[0] == stored[8192]
+    for bad in (-64, 100):
+        with pytest.raises(gpu.NpbcgError) as e:
+            gpu.Context(m, sort_window=bad)
+        assert e.value.code == gpu.ERR_INVALID
+
+
 @pytest.mark.parametrize("cg_form,dot_pass", [(1, 0), (2, 1), (2, 2)])
 def test_cg_forms_dropin_vectors(gpu, golden, cg_form, dot_pass):
     c = golden["cfg1_n1400_seed12345"]
