@@ -633,7 +633,8 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
                 : std::min(kMaxSortWindow, std::max(npbcg_detail::kSortWindow,
                                                     (R + kSlice - 1) / kSlice * kSlice));
         if (W * 8 > lds_budget || !sorted ||
-            !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, &sm, window)) {
+            !npbcg_detail::build_sell_blocks(n, rowstr, colidx, a, W, R, &sm, window,
+                                             block / 64)) {
             variant = pick_variant(npbcg_dev::kSpmvSeg64, n, nnz, false);
         } else {
             c->sort_window = window;

@@ -188,10 +188,11 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell_kernel(
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int s0 = wg_slice[wg];
     const int nsl = wg_slice[wg + 1] - s0;
-    // Slice (index in this workgroup) of round rd for this wave, or -1.
-    // Slices are sorted widest first; alternating the direction of each round
-    // ("snake" order) evens out the total width per wavefront.  The index
-    // grows with rd, so after the first -1 there is none.
+    // Slice (index in this workgroup) of round rd for this wave, or -1, in
+    // "snake" order (the direction alternates by round).  The layout places
+    // the slices so that every wavefront's total width under this order is
+    // close to the others' (build_sell_blocks).  The index grows with rd, so
+    // after the first -1 there is none.
     auto slice_of = [&](int rd) {
         const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
         return k < nsl ? k : -1;

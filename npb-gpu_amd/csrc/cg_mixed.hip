@@ -80,8 +80,9 @@ __global__ __launch_bounds__(BLOCK) void spmv_sell32_kernel(
     const int s0 = wg_slice[wg];
     const int nsl = wg_slice[wg + 1] - s0;
     float *rows_lds = reinterpret_cast<float *>(slice_dot + nsl);  // stage_rows entries
-    // Slice of round rd for this wave (snake order, widest first), or -1;
-    // the index grows with rd, so after the first -1 there is none.
+    // Slice of round rd for this wave (snake order, slices placed for even
+    // widths per wavefront), or -1; the index grows with rd, so after the
+    // first -1 there is none.
     auto slice_of = [&](int rd) {
         const int k = rd * kWaves + ((rd & 1) ? kWaves - 1 - wave : wave);
         return k < nsl ? k : -1;

@@ -3,10 +3,10 @@
 // inside a larger kernel.
 //
 // Workgroup wg owns column block b = wg / nrange of row range wg % nrange and
-// has nsl slices of 64 rows, widest first.  Wavefront w takes slice
-// rd * waves + (w or waves-1-w) in round rd ("snake" order, which evens out
-// the total width per wavefront) and walks its rounds forward or, with rev,
-// backward: successive SpMVs alternate so each starts on the slices the
+// has nsl slices of 64 rows.  Wavefront w takes slice rd * waves + (w or
+// waves-1-w) in round rd ("snake" order; the layout places the slices so
+// that the wavefronts' total widths are close) and walks its rounds forward
+// or, with rev, backward: successive SpMVs alternate so each starts on the slices the
 // previous one streamed last, which the memory-side cache still holds.  Lane
 // l accumulates its own row down the slice in entry order, so neither the
 // order nor the direction changes any sum.

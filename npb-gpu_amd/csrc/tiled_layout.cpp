@@ -57,9 +57,43 @@ void block_segments(int n, const int *rowstr, const int *colidx, int W, int R, i
     }
 }
 
+// Position of each slice of a workgroup (slices in sorted order, widths
+// given).  The kernels give position k of round rd = k / waves to wavefront
+// (rd even) k % waves or (rd odd) waves - 1 - k % waves.  With waves > 0 the
+// slices are placed so that the wavefronts' total widths are close: widest
+// first, each to the wavefront with the least width so far among those with
+// a position left (every wavefront keeps the number of positions the rounds
+// give it).  waves = 0 keeps the sorted order.
+void place_slices(const std::vector<int> &width, int waves, std::vector<int> *pos) {
+    const int nsl = (int)width.size();
+    pos->resize(nsl);
+    if (waves <= 1 || nsl <= waves) {
+        for (int i = 0; i < nsl; i++) (*pos)[i] = i;
+        return;
+    }
+    std::vector<std::vector<int>> slots(waves);  // positions of each wavefront, in order
+    for (int k = 0; k < nsl; k++) {
+        const int rd = k / waves, i = k % waves;
+        slots[(rd & 1) ? waves - 1 - i : i].push_back(k);
+    }
+    std::vector<int> order(nsl);
+    for (int i = 0; i < nsl; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return width[x] > width[y]; });
+    std::vector<long long> load(waves, 0);
+    std::vector<int> used(waves, 0);
+    for (int i : order) {
+        int best = -1;
+        for (int w = 0; w < waves; w++)
+            if (used[w] < (int)slots[w].size() && (best < 0 || load[w] < load[best])) best = w;
+        (*pos)[i] = slots[best][used[best]++];
+        load[best] += width[i] + 2;  // + the per-slice start
+    }
+}
+
 template <class V>
 bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, int W, int R,
-                int window, SellBlocks<V> *out) {
+                int window, int waves, SellBlocks<V> *out) {
     if (n < 1 || W < 1 || W > 65536 || R < 1 || window < kSliceRows || window % kSliceRows)
         return false;
     SellBlocks<V> &m = *out;
@@ -107,9 +141,16 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
         const int b = wg / m.nrange, rr = wg % m.nrange;
         block_segments(n, rowstr, colidx, W, R, b, rr, window, &segs);
         long long off = wg_base[wg];
-        int s = m.wg_slice[wg];
-        for (size_t i0 = 0; i0 < segs.size(); i0 += kSliceRows, s++) {
-            const int width = segs[i0].hi - segs[i0].lo;
+        const int nsl = m.wg_slice[wg + 1] - m.wg_slice[wg];
+        std::vector<int> width(nsl), pos, at_pos(nsl);
+        for (int i = 0; i < nsl; i++)
+            width[i] = segs[(size_t)i * kSliceRows].hi - segs[(size_t)i * kSliceRows].lo;
+        place_slices(width, waves, &pos);
+        for (int i = 0; i < nsl; i++) at_pos[pos[i]] = i;
+        for (int k = 0; k < nsl; k++) {
+            const int s = m.wg_slice[wg] + k;
+            const size_t i0 = (size_t)at_pos[k] * kSliceRows;
+            const int width_k = width[at_pos[k]];
             m.slice_off[s] = (int)off;
             for (int l = 0; l < kSliceRows && i0 + l < segs.size(); l++) {
                 const RowSeg &rs = segs[i0 + l];
@@ -121,7 +162,7 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
                     m.col[at] = (uint16_t)(colidx[rs.lo + j] - b * W);
                 }
             }
-            off += (long long)kSliceRows * width;
+            off += (long long)kSliceRows * width_k;
         }
         if (wg == nwg - 1) m.slice_off[nslices] = (int)off;
     });
@@ -132,13 +173,13 @@ bool build_impl(int n, const int *rowstr, const int *colidx, const double *a, in
 }  // namespace
 
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
-                       int R, SellBlocks<double> *out, int window) {
-    return build_impl(n, rowstr, colidx, a, W, R, window, out);
+                       int R, SellBlocks<double> *out, int window, int waves) {
+    return build_impl(n, rowstr, colidx, a, W, R, window, waves, out);
 }
 
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a, int W,
-                       int R, SellBlocks<float> *out, int window) {
-    return build_impl(n, rowstr, colidx, a, W, R, window, out);
+                       int R, SellBlocks<float> *out, int window, int waves) {
+    return build_impl(n, rowstr, colidx, a, W, R, window, waves, out);
 }
 
 }  // namespace npbcg_detail

@@ -43,16 +43,21 @@ struct SellBlocks {
 };
 
 // Rows are sorted by segment length within windows of `window` rows (a
-// multiple of 64; kSortWindow unless the caller widens it).
+// multiple of 64; kSortWindow unless the caller widens it).  With waves > 0
+// (wavefronts per workgroup) a workgroup's slices are placed so that the
+// wavefronts of the kernels' round order get close total widths; with 0 they
+// stay in sorted order (widest first).
 constexpr int kSortWindow = 2048;
 
 // Builds the copy; the CSR must have ascending columns within each row.
 // Returns false if W > 65536, the window is not a positive multiple of 64
 // or the sizes overflow 32-bit offsets.
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
-                       int W, int R, SellBlocks<double> *out, int window = kSortWindow);
+                       int W, int R, SellBlocks<double> *out, int window = kSortWindow,
+                       int waves = 0);
 bool build_sell_blocks(int n, const int *rowstr, const int *colidx, const double *a,
-                       int W, int R, SellBlocks<float> *out, int window = kSortWindow);
+                       int W, int R, SellBlocks<float> *out, int window = kSortWindow,
+                       int waves = 0);
 
 }  // namespace npbcg_detail
 
