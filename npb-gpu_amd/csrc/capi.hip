@@ -620,7 +620,28 @@ int npbcg_create(npbcg_ctx **out, int n, const int *rowstr, const int *colidx,
             }
             if (c->opts.tile_rows <= 0) {
                 const int rmax = kSlice * (lds_budget / 8 - W - 1);
-                if (rmax >= kSlice) R = std::min(R, rmax);
+                if (rmax >= kSlice && R > rmax) {
+                    // The capped ranges give nblk * nrange workgroups for
+                    // `slots` CU slots; of the range counts from the least
+                    // the cap allows up to twice that, take the one whose
+                    // last round of workgroups fills the most slots (equal
+                    // workgroups: a part-filled round leaves CUs idle).  At
+                    // 1.5 M rows: 85 blocks x 12 ranges = 3.98 rounds of 256
+                    // instead of 85 x 10 = 3.32.
+                    const int slots = cus * (block == 512 ? 2 : 1);
+                    const int nr0 = (n + rmax - 1) / rmax;
+                    int best = nr0;
+                    double best_idle = 2.0;
+                    for (int nr = nr0; nr <= 2 * nr0; nr++) {
+                        const double rounds = (double)nblk * nr / slots;
+                        const double idle = (std::ceil(rounds) - rounds) / std::ceil(rounds);
+                        if (idle < best_idle - 1e-9) {
+                            best_idle = idle;
+                            best = nr;
+                        }
+                    }
+                    R = (int)(((long long)n + best - 1) / best);
+                }
             }
         }
         // Rows are sorted by their length in a block over the whole range,
