@@ -271,6 +271,31 @@ def test_cg_forms_match_reference(gpu, golden, name, cg_form, dot_pass):
     assert np.all(np.abs(rnorm - rref) <= 1e-12 * np.sqrt(m.n))
 
 
+def test_capped_range_shape_matches_oracle(gpu):
+    # At n = 1,000,000 one row range's per-slice p.q slots no longer fit in
+    # the LDS beside a full vector block, so the default shape narrows the
+    # block and caps the range, then picks the range count whose last round
+    # of workgroups fills the most CU slots (the class D path, here with a
+    # sparse matrix the oracle runs in seconds).  The power loop on that
+    # shape is held to the oracle's zeta and rnorm.  (With 13 entries per row
+    # the automatic choice would take a CSR row kernel, so the layout is named.)
+    m = gpu.makea(1000000, 3, 10.0, 314159265.0)
+    niter = 2
+    with gpu.Context(m, spmv_variant=5) as ctx:
+        d = ctx.describe()
+        zeta, rnorm = ctx.power(niter, m.shift)
+    print(d)
+    assert d["spmv_variant"] == 5
+    nwg = d["col_blocks"] * d["row_ranges"]
+    assert d["sell_workgroups"] == nwg and d["range_rows"] * d["row_ranges"] >= m.n
+    # the capped ranges hold at most the slices the LDS leaves beside the block
+    assert (d["block_width"] + (d["range_rows"] + 63) // 64) * 8 <= 160 * 1024
+    assert d["sort_window"] == 8192
+    ozeta, ornorm, _ = oracle.power(_ocsr(m), niter)
+    _zeta_close(zeta, ozeta, "capped range shape")
+    assert np.all(np.abs(rnorm - ornorm) <= 1e-12 * np.sqrt(m.n) * np.maximum(1.0, np.abs(ornorm)))
+
+
 @pytest.mark.parametrize("window", [64, 2048, 8192])
 def test_sort_windows_match_reference(gpu, golden, window):
     # Rows are sorted by their length in a column block within sort windows
